@@ -1,0 +1,60 @@
+// Launch-argument structures shared by the host session and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "models.hpp"
+
+namespace cubnm {
+
+// Everything the integrator kernels read or write for one batch of
+// simulations. Layouts follow the reference boundary (src/ext/core.cpp:336-386):
+//   gp    [n_global][N_SIMS]            rp    [n_regional][N_SIMS*nodes]
+//   SC    [N_SCs][source k][target j]   bold  [N_SIMS][bold_len][nodes]
+//   states[n_state][N_SIMS][nodes or states_len*nodes]
+//   gbool [n_gout_bool][N_SIMS]         gint  [n_gout_int][N_SIMS]
+struct IntegrateArgs {
+    int N_SIMS, nodes;
+    int bw_it, inner_it, BOLD_TR_iters, states_sampling_iters, n_states_samples_remove;
+    int noise_bw_it, states_ts, bold_len, states_len, noise_repeats;
+    int do_delay;
+    double dt;
+    const double* SC;
+    const int* SC_indices;
+    const double* gp;
+    double* rp;
+    const double* noise;      // noise segment [noise_bw_it*inner_it][nodes][n_noise]
+    const int* sh_nodes;      // [noise_repeats][nodes]
+    const int* sh_ts;         // [noise_repeats][noise_bw_it]
+    double* bold;
+    double* states;
+    unsigned char* gbool;
+    int* gint;
+    // conduction delay (general kernel only)
+    const int* delays;        // [N_SIMS][nodes][nodes] (k, j), or null
+    const int* max_delays;    // [N_SIMS]
+    double* hist;             // circular history, hist_stride doubles per sim
+    long long hist_stride;
+    BWConsts bw;
+};
+
+struct FcArgs {
+    int N_SIMS, nodes, bold_len, n_vols_remove, n_pairs, n_windows, n_window_pairs;
+    int exc_inter;
+    const double* bold;        // [N_SIMS][bold_len][nodes]
+    const int* pairs_i;        // [n_pairs]
+    const int* pairs_j;
+    const int* w_starts;       // [n_windows], inclusive
+    const int* w_ends;         // [n_windows], inclusive
+    const int* wpairs_i;       // [n_window_pairs]
+    const int* wpairs_j;
+    double* fc;                // [N_SIMS][n_pairs]
+    double* fcd;               // [N_SIMS][n_window_pairs]
+    double* wfc_scratch;       // [n_slots][n_windows][n_pairs]
+    int n_slots;
+};
+
+// Host-side launchers (defined in the .hip files).
+hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* model_consts, hipStream_t s);
+hipError_t launch_fc_fcd(const FcArgs& a, hipStream_t s);
+
+}  // namespace cubnm

@@ -1,0 +1,166 @@
+// Neural-mass model definitions shared by the host (constants, analytical
+// FIC) and the gfx950 integrator kernels (per-node Euler-Maruyama steps).
+//
+// Counts, state layouts and update expressions follow the reference models:
+//   rWW       include/cubnm/models/rww.hpp:88-109, src/ext/models/rww.cpp:16-53, 149-313
+//   JR        include/cubnm/models/jr.hpp:55-75,   src/ext/models/jr.cpp:14-133
+//   Kuramoto  include/cubnm/models/kuramoto.hpp,   src/ext/models/kuramoto.cpp:14-61
+//   WC        include/cubnm/models/wc.hpp:50-70,   src/ext/models/wc.cpp:14-80
+// The expressions keep the reference's operation order (left-to-right
+// products such as (w_p*J_N)*S_E) so that FP64 results agree with the CPU
+// path to rounding; noise enters as already-gathered values.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+#define CUBNM_HD __host__ __device__ inline
+
+namespace cubnm {
+
+enum ModelId { MODEL_RWW = 0, MODEL_JR = 1, MODEL_KURAMOTO = 2, MODEL_WC = 3, MODEL_COUNT = 4 };
+
+// Balloon-Windkessel constants (src/ext/models/bw.cpp:4-22; core.cpp:157-192, 291-292)
+struct BWConsts {
+    double dt, rho, alpha, tau, y, kappa, V_0, k1, k2, k3;
+    double ialpha, itau, oneminrho, dt_itau;
+};
+
+// rWW ---------------------------------------------------------------------
+struct RWW {
+    static constexpr int n_state = 6, n_noise = 2, n_global = 1, n_regional = 4;
+    static constexpr int conn_idx = 4, bold_idx = 4, n_gout_bool = 2, n_gout_int = 1;
+    static constexpr bool is_osc = false;
+    struct C {
+        double dt, sqrt_dt, a_E, b_E, d_E, a_I, b_I, d_I, w_II;
+        double w_E__I_0, w_I__I_0, b_a_ratio_E, dt_itau_E, dt_gamma_E, dt_itau_I, dt_gamma_I;
+        // numerical FIC config (rww.hpp:61-69)
+        int do_fic, max_fic_trials, I_SAMPLING_START, I_SAMPLING_END, I_SAMPLING_DURATION;
+        double init_delta;
+        // analytical FIC constants
+        double tau_I_s, gamma_I_s, I_0, w_E, w_I, I_ext, I_E_ss, S_E_ss;
+    };
+    static void init_constants(C& c, double dt) {
+        c.dt = dt; c.sqrt_dt = std::sqrt(dt);
+        c.a_E = 310; c.b_E = 125; c.d_E = 0.16; c.a_I = 615; c.b_I = 177; c.d_I = 0.087;
+        const double gamma_E = (double)0.641 / (double)1000.0, gamma_I = (double)1.0 / (double)1000.0;
+        const double tau_E = 100, tau_I = 10;
+        c.I_0 = 0.382; c.w_E = 1.0; c.w_I = 0.7; c.w_II = 1.0; c.I_ext = 0.0;
+        c.w_E__I_0 = c.w_E * c.I_0; c.w_I__I_0 = c.w_I * c.I_0; c.b_a_ratio_E = c.b_E / c.a_E;
+        c.dt_itau_E = dt * (1.0 / tau_E); c.dt_gamma_E = dt * gamma_E;
+        c.dt_itau_I = dt * (1.0 / tau_I); c.dt_gamma_I = dt * gamma_I;
+        c.tau_I_s = 0.01; c.gamma_I_s = 1.0;
+        c.I_E_ss = 0.3773805650; c.S_E_ss = 0.1647572075;
+        c.do_fic = 1; c.max_fic_trials = 0; c.I_SAMPLING_START = 1000; c.I_SAMPLING_END = 10000;
+        c.I_SAMPLING_DURATION = 10000 - 1000 + 1; c.init_delta = 0.02;
+    }
+    CUBNM_HD static void init(const C&, double* s, const double*) { s[4] = 0.001; s[5] = 0.001; }
+    // rww.cpp:187-225
+    CUBNM_HD static void step(const C& c, double* s, const double* gp, const double* rp, double g,
+                              double n0, double n1) {
+        s[0] = c.w_E__I_0 + rp[0] * rp[1] * s[4] + g * gp[0] * rp[1] - rp[2] * s[5];
+        s[1] = c.w_I__I_0 + rp[1] * s[4] - c.w_II * s[5];
+        const double aE = c.a_E * s[0] - c.b_E;
+        const double aI = c.a_I * s[1] - c.b_I;
+        s[2] = aE / (1 - exp(-c.d_E * aE));
+        s[3] = aI / (1 - exp(-c.d_I * aI));
+        const double dSE = n0 * rp[3] * c.sqrt_dt + c.dt_gamma_E * ((1 - s[4]) * s[2]) - c.dt_itau_E * s[4];
+        const double dSI = n1 * rp[3] * c.sqrt_dt + c.dt_gamma_I * s[3] - c.dt_itau_I * s[5];
+        s[4] += dSE;
+        s[5] += dSI;
+        s[4] = fmax(0.0, fmin(1.0, s[4]));
+        s[5] = fmax(0.0, fmin(1.0, s[5]));
+    }
+};
+
+// Jansen-Rit ----------------------------------------------------------------
+struct JR {
+    static constexpr int n_state = 8, n_noise = 1, n_global = 1, n_regional = 6;
+    static constexpr int conn_idx = 7, bold_idx = 6, n_gout_bool = 0, n_gout_int = 0;
+    static constexpr bool is_osc = false;
+    struct C { double dt, sqrt_dt, A, B, a, b, v0, nu_max, r, mu, cmin, cmax, midpoint, a2, b2; };
+    static void init_constants(C& c, double dt) {
+        c.dt = dt; c.sqrt_dt = std::sqrt(dt);
+        c.A = 3.25; c.B = 22.0; c.a = 0.1; c.b = 0.05; c.v0 = 6.0; c.nu_max = 0.0025; c.r = 0.56;
+        c.mu = 0.22; c.cmin = 0.0; c.cmax = 0.005; c.midpoint = 6.0;
+        c.a2 = std::pow(c.a, 2); c.b2 = std::pow(c.b, 2);  // the reference evaluates pow(a, 2) per step
+    }
+    CUBNM_HD static void init(const C&, double* s, const double*) { for (int i = 0; i < 8; i++) s[i] = 0.0; }
+    // jr.cpp:80-133
+    CUBNM_HD static void step(const C& c, double* s, const double* gp, const double* rp, double g,
+                              double n0, double) {
+        const double sg0 = 2.0 * c.nu_max / (1.0 + exp(c.r * (c.v0 - (s[1] - s[2]))));
+        const double sg1 = 2.0 * c.nu_max / (1.0 + exp(c.r * (c.v0 - (rp[1] * rp[0] * s[0]))));
+        const double sg2 = 2.0 * c.nu_max / (1.0 + exp(c.r * (c.v0 - (rp[3] * rp[0] * s[0]))));
+        const double d0 = s[3], d1 = s[4], d2 = s[5];
+        const double d3 = c.A * c.a * sg0 - 2.0 * c.a * s[3] - c.a2 * s[0];
+        const double d4 = c.A * c.a * (c.mu + rp[2] * rp[0] * sg1 + gp[0] * g) - 2.0 * c.a * s[4] - c.a2 * s[1];
+        const double d5 = c.B * c.b * (rp[4] * rp[0] * sg2) - 2.0 * c.b * s[5] - c.b2 * s[2];
+        s[0] += c.dt * d0;
+        s[1] += c.dt * d1;
+        s[2] += c.dt * d2;
+        s[3] += c.dt * d3 + c.sqrt_dt * n0 * rp[5];
+        s[4] += c.dt * d4;
+        s[5] += c.dt * d5;
+        s[6] = s[1] - s[2];
+        s[7] = c.cmin + (c.cmax - c.cmin) / (1.0 + exp(c.r * (c.midpoint - (s[6]))));
+    }
+};
+
+// Kuramoto ------------------------------------------------------------------
+struct Kuramoto {
+    static constexpr int n_state = 1, n_noise = 1, n_global = 1, n_regional = 3;
+    static constexpr int conn_idx = 0, bold_idx = 0, n_gout_bool = 0, n_gout_int = 0;
+    static constexpr bool is_osc = true;
+    struct C { double dt, sqrt_dt, twopi; };
+    static void init_constants(C& c, double dt) { c.dt = dt; c.sqrt_dt = std::sqrt(dt); c.twopi = 2.0 * M_PI; }
+    CUBNM_HD static void init(const C&, double* s, const double* rp) { s[0] = rp[0]; }
+    // kuramoto.cpp:48-61
+    CUBNM_HD static void step(const C& c, double* s, const double* gp, const double* rp, double g,
+                              double n0, double) {
+        const double td = rp[1] + gp[0] * g;
+        s[0] = s[0] + c.dt * td + c.sqrt_dt * n0 * rp[2];
+        s[0] = fmod(s[0], c.twopi);
+    }
+};
+
+// Wilson-Cowan ----------------------------------------------------------------
+struct WC {
+    static constexpr int n_state = 2, n_noise = 2, n_global = 1, n_regional = 8;
+    static constexpr int conn_idx = 0, bold_idx = 0, n_gout_bool = 0, n_gout_int = 0;
+    static constexpr bool is_osc = false;
+    struct C { double dt, sqrt_dt, itau_E, itau_I, a_E, a_I, mu_E, mu_I; };
+    static void init_constants(C& c, double dt) {
+        c.dt = dt; c.sqrt_dt = std::sqrt(dt);
+        c.itau_E = 1.0 / 2.5; c.itau_I = 1.0 / 3.75; c.a_E = 1.5; c.a_I = 1.5; c.mu_E = 3.0; c.mu_I = 3.0;
+    }
+    CUBNM_HD static void init(const C&, double* s, const double*) { s[0] = 0.001; s[1] = 0.001; }
+    // wc.cpp:51-80
+    CUBNM_HD static void step(const C& c, double* s, const double* gp, const double* rp, double g,
+                              double n0, double n1) {
+        const double SE = 1.0 / (1.0 + exp(-c.a_E * (rp[0] * s[0] - rp[1] * s[1] + gp[0] * g + rp[4] - c.mu_E)));
+        const double SI = 1.0 / (1.0 + exp(-c.a_I * (rp[2] * s[0] - rp[3] * s[1] + rp[5] - c.mu_I)));
+        const double dE = (-s[0] + (1.0 - s[0]) * SE) * c.itau_E;
+        const double dI = (-s[1] + (1.0 - s[1]) * SI) * c.itau_I;
+        s[0] += c.dt * dE + c.sqrt_dt * n0 * rp[6];
+        s[1] += c.dt * dI + c.sqrt_dt * n1 * rp[7];
+        s[0] = fmax(0.0, fmin(1.0, s[0]));
+        s[1] = fmax(0.0, fmin(1.0, s[1]));
+    }
+};
+
+// One Balloon-Windkessel Euler step (bw.cpp:24-34): nu and q use the updated
+// nu, both use the old f, and f advances with the updated x.
+CUBNM_HD void bw_step(const BWConsts& b, double s, double& x, double& f, double& nu, double& q) {
+    x = x + b.dt * (s - b.kappa * x - b.y * (f - 1.0));
+    const double tmp_f = f + b.dt * x;
+    nu = nu + b.dt_itau * (f - pow(nu, b.ialpha));
+    q = q + b.dt_itau * (f * (1.0 - pow(b.oneminrho, (1.0 / f))) / b.rho - pow(nu, b.ialpha) * q / nu);
+    f = tmp_f;
+}
+
+// BOLD observation in the CPU engine's form (bnm.cpp:318).
+CUBNM_HD double bold_signal(const BWConsts& b, double nu, double q) {
+    return b.V_0 * (b.k1 * (1 - q) + b.k2 * (1 - q / nu) + b.k3 * (1 - nu));
+}
+
+}  // namespace cubnm

@@ -1,0 +1,736 @@
+// Host side of the MI355X simulation core: the process-global session behind
+// the C ABI in include/cubnm_amd.h.
+//
+// Mirrors the reference boundary src/ext/core.cpp:147-504 and the host parts
+// of its GPU engine (src/ext/bnm.cu:507-1265): model/session selection,
+// string config, loop-count arithmetic (include/cubnm/models/base.hpp:51-68),
+// FC pair and FCD window tables (src/ext/utils.cpp:6-72), host noise
+// generation with std::mt19937 + std::normal_distribution + std::shuffle
+// (bnm.cu:1207-1263, utils.cpp:74-94), rWW analytical FIC
+// (src/ext/models/rww.cpp:73-146, rww_fic.cpp:105-231), per-simulation
+// conduction delays (bnm.cpp:146-216), then the HIP kernels.
+#include "engine.hpp"
+#include "../../include/cubnm_amd.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace cubnm {
+
+hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
+                                    int* launches);
+bool batched_supported(int model, int nodes);
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- BW constants
+BWConsts g_bw;
+bool g_bw_ready = false;
+
+void bw_defaults(BWConsts& b) {
+    b.dt = 0.001; b.rho = 0.34; b.alpha = 0.32; b.tau = 0.98; b.y = 1.0 / 0.41; b.kappa = 1.0 / 0.65;
+    b.V_0 = 0.02 * 100; b.k1 = 7 * b.rho; b.k2 = 2.0; b.k3 = 2 * b.rho - 0.2;
+    b.ialpha = 1.0 / b.alpha; b.itau = 1.0 / b.tau; b.oneminrho = 1.0 - b.rho; b.dt_itau = b.dt * b.itau;
+}
+
+// ---------------------------------------------------------------- device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n) {
+        if (n <= bytes && p) return;
+        if (p) check(hipFree(p), "hipFree");
+        p = nullptr; bytes = 0;
+        if (n == 0) return;
+        check(hipMalloc(&p, n), "hipMalloc");
+        bytes = n;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------- model table
+struct ModelInfo {
+    const char* name; int id, n_state, n_global, n_regional, n_noise, n_gb, n_gi;
+};
+const ModelInfo kModels[] = {
+    {"rWW", MODEL_RWW, RWW::n_state, RWW::n_global, RWW::n_regional, RWW::n_noise, RWW::n_gout_bool, RWW::n_gout_int},
+    {"JR", MODEL_JR, JR::n_state, JR::n_global, JR::n_regional, JR::n_noise, JR::n_gout_bool, JR::n_gout_int},
+    {"Kuramoto", MODEL_KURAMOTO, Kuramoto::n_state, Kuramoto::n_global, Kuramoto::n_regional, Kuramoto::n_noise, 0, 0},
+    {"WC", MODEL_WC, WC::n_state, WC::n_global, WC::n_regional, WC::n_noise, 0, 0},
+};
+
+const ModelInfo* find_model(const char* name) {
+    for (const auto& m : kModels) if (std::strcmp(m.name, name) == 0) return &m;
+    return nullptr;
+}
+
+// ---------------------------------------------------------------- config
+struct BaseConf {
+    bool do_fc = true, do_fcd = true;
+    int bold_remove_s = 30;
+    bool exc_interhemispheric = true;
+    int window_size = 10, window_step = 2;
+    bool drop_edges = true;
+    int noise_time_steps = 30000;
+    bool ext_out = true, states_ts = false;
+};
+
+// ---------------------------------------------------------------- session
+struct Session {
+    const ModelInfo* model = nullptr;
+    bool initialized = false;
+    // run shape
+    int N_SIMS = 0, nodes = 0, N_SCs = 0, BOLD_TR = 0, states_sampling = 0, time_steps = 0, sim_seed = 0;
+    bool do_delay = false, noise_out = false;
+    double dt = 0.1, bw_dt = 0.001;  // ms, s
+    BaseConf conf;
+    bool modifies_params = false;
+    RWW::C rww; JR::C jr; Kuramoto::C kur; WC::C wc;
+    // derived
+    int bold_len = 0, bold_size = 0, BOLD_TR_iters = 0, states_len = 0, states_sampling_iters = 0;
+    int bw_it = 0, inner_it = 0, n_states_samples_remove = 0, n_vols_remove = 0, corr_len = 0;
+    int n_pairs = 0, n_windows = 0, n_window_pairs = 0;
+    std::vector<int> w_starts, w_ends, pairs_i, pairs_j, wpairs_i, wpairs_j;
+    // noise cache
+    int noise_bw_it = 0, noise_repeats = 0;
+    long long noise_size = 0;
+    int last_seed = -1, last_time_steps = -1, last_nodes = -1, last_noise_ts = -1, last_n_noise = -1;
+    bool noise_ready = false;
+    std::vector<double> noise;
+    std::vector<int> sh_nodes, sh_ts;
+    // device buffers
+    DevBuf d_noise, d_shn, d_sht, d_pairs, d_wins, d_wpairs;
+    DevBuf d_SC, d_SCi, d_gp, d_rp, d_v, d_bold, d_fc, d_fcd, d_states, d_gb, d_gi, d_delays, d_maxd, d_hist, d_wfc;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    double timings[6] = {0, 0, 0, 0, 0, 0};
+    int force_integrator = -1, last_integrator = -1;
+};
+
+Session g_s;
+
+int parse_int(const std::string& v) { return std::stoi(v); }
+
+void apply_config(Session& s, const std::vector<std::pair<std::string, std::string>>& kv) {
+    s.conf = BaseConf();
+    for (const auto& p : kv) {
+        const std::string& k = p.first; const std::string& v = p.second;
+        if (k == "do_fc") s.conf.do_fc = (bool)parse_int(v);
+        else if (k == "do_fcd") s.conf.do_fcd = (bool)parse_int(v);
+        else if (k == "bold_remove_s") s.conf.bold_remove_s = parse_int(v);
+        else if (k == "exc_interhemispheric") s.conf.exc_interhemispheric = (bool)parse_int(v);
+        else if (k == "window_size") s.conf.window_size = parse_int(v);
+        else if (k == "window_step") s.conf.window_step = parse_int(v);
+        else if (k == "drop_edges") s.conf.drop_edges = (bool)parse_int(v);
+        else if (k == "noise_time_steps") s.conf.noise_time_steps = parse_int(v);
+        // verbose / progress_interval are accepted and ignored (no progress polling)
+    }
+    s.modifies_params = false;
+    if (s.model->id == MODEL_RWW) {
+        // rww.cpp:54-71; the duration is recomputed after every key, as upstream
+        for (const auto& p : kv) {
+            const std::string& k = p.first; const std::string& v = p.second;
+            if (k == "do_fic") { s.rww.do_fic = (bool)parse_int(v); s.modifies_params = true; }
+            else if (k == "max_fic_trials") s.rww.max_fic_trials = parse_int(v);
+            else if (k == "I_SAMPLING_START") s.rww.I_SAMPLING_START = (int)((parse_int(v) / 1000) / s.bw_dt);
+            else if (k == "I_SAMPLING_END") s.rww.I_SAMPLING_END = (int)((parse_int(v) / 1000) / s.bw_dt);
+            else if (k == "init_delta") s.rww.init_delta = std::stod(v);
+            s.rww.I_SAMPLING_DURATION = s.rww.I_SAMPLING_END - s.rww.I_SAMPLING_START + 1;
+        }
+    }
+}
+
+// base.hpp:51-68 and bnm.cpp:631-657: double division, then truncation
+void derive_shape(Session& s) {
+    s.bold_len = s.time_steps / s.BOLD_TR;
+    s.bold_size = s.bold_len * s.nodes;
+    s.BOLD_TR_iters = (int)(((double)s.BOLD_TR / 1000.0) / s.bw_dt);
+    s.states_len = s.time_steps / s.states_sampling;
+    s.states_sampling_iters = (int)(((double)s.states_sampling / 1000.0) / s.bw_dt);
+    s.bw_it = (int)(((double)s.time_steps / 1000.0) / s.bw_dt);
+    s.inner_it = (int)((s.bw_dt * 1000) / s.dt);
+    if (s.bw_it <= 0 || s.inner_it <= 0 || s.BOLD_TR_iters <= 0 || s.states_sampling_iters <= 0)
+        throw std::runtime_error("invalid timing: duration, TR, states_sampling, dt and bw_dt give a zero loop count");
+    s.n_states_samples_remove = s.conf.bold_remove_s * 1000 / s.states_sampling;
+    s.n_pairs = s.n_windows = s.n_window_pairs = 0;
+    s.pairs_i.clear(); s.pairs_j.clear(); s.w_starts.clear(); s.w_ends.clear();
+    s.wpairs_i.clear(); s.wpairs_j.clear();
+    if (!s.conf.do_fc) return;
+    s.n_vols_remove = s.conf.bold_remove_s * 1000 / s.BOLD_TR;
+    s.corr_len = s.bold_len - s.n_vols_remove;
+    if (s.corr_len < 2)
+        throw std::runtime_error("Number of BOLD volumes (after removing initial volumes) is too low for FC calculations");
+    // utils.cpp:6-19 and bnm.cu:1093-1122
+    const int N = s.nodes;
+    int rh = N / 2;
+    if (s.conf.exc_interhemispheric && (N % 2) != 0)
+        throw std::runtime_error("exc_interhemispheric is set but number of nodes is not even");
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < i; j++) {
+            if (s.conf.exc_interhemispheric && ((i < rh) ^ (j < rh))) continue;
+            s.pairs_i.push_back(i); s.pairs_j.push_back(j);
+        }
+    s.n_pairs = (int)s.pairs_i.size();
+    if (!s.conf.do_fcd) return;
+    // utils.cpp:21-72
+    int first, last;
+    if (s.conf.drop_edges) { first = s.conf.window_size / 2; last = s.corr_len - 1 - (s.conf.window_size / 2); }
+    else { first = 0; last = s.corr_len - 1; }
+    first += s.n_vols_remove; last += s.n_vols_remove;
+    if (s.conf.window_step <= 0) throw std::runtime_error("window_step must be positive");
+    for (int c = first; c <= last; c += s.conf.window_step) {
+        int ws = std::max(c - s.conf.window_size / 2, 0);
+        int we = std::min(c + s.conf.window_size / 2, s.bold_len - 1);
+        s.w_starts.push_back(ws); s.w_ends.push_back(we);
+    }
+    s.n_windows = (int)s.w_starts.size();
+    if (s.n_windows == 0) throw std::runtime_error("Number of dynamic FC windows is 0");
+    for (int i = 0; i < s.n_windows; i++)
+        for (int j = 0; j < i; j++) { s.wpairs_i.push_back(i); s.wpairs_j.push_back(j); }
+    s.n_window_pairs = (int)s.wpairs_i.size();
+}
+
+void init_model_constants(Session& s) {
+    // Config values survive this call: keep them aside and restore.
+    RWW::C keep = s.rww;
+    RWW::init_constants(s.rww, s.dt);
+    s.rww.do_fic = keep.do_fic; s.rww.max_fic_trials = keep.max_fic_trials;
+    s.rww.I_SAMPLING_START = keep.I_SAMPLING_START; s.rww.I_SAMPLING_END = keep.I_SAMPLING_END;
+    s.rww.I_SAMPLING_DURATION = keep.I_SAMPLING_DURATION; s.rww.init_delta = keep.init_delta;
+    JR::init_constants(s.jr, s.dt);
+    Kuramoto::init_constants(s.kur, s.dt);
+    WC::init_constants(s.wc, s.dt);
+}
+
+const void* model_consts(const Session& s) {
+    switch (s.model->id) {
+        case MODEL_RWW: return &s.rww;
+        case MODEL_JR: return &s.jr;
+        case MODEL_KURAMOTO: return &s.kur;
+        default: return &s.wc;
+    }
+}
+
+// Noise segment and per-repeat shuffles from one std::mt19937 stream.
+void ensure_noise(Session& s, bool force) {
+    const int n_noise = s.model->n_noise;
+    s.noise_bw_it = (int)(((double)(s.conf.noise_time_steps) / 1000.0) / s.bw_dt);
+    if (s.noise_bw_it <= 0) throw std::runtime_error("noise segment shorter than one BW step");
+    s.noise_size = (long long)s.nodes * s.noise_bw_it * s.inner_it * n_noise;
+    s.noise_repeats = (int)std::ceil((double)(s.bw_it) / (double)(s.noise_bw_it));
+    const bool stale = !s.noise_ready || force || s.sim_seed != s.last_seed || s.time_steps != s.last_time_steps ||
+                       s.nodes != s.last_nodes || s.conf.noise_time_steps != s.last_noise_ts ||
+                       n_noise != s.last_n_noise;
+    if (!stale) return;
+    std::mt19937 gen(s.sim_seed);
+    std::normal_distribution<double> nd(0, 1);
+    s.noise.resize(s.noise_size);
+    for (long long i = 0; i < s.noise_size; i++) s.noise[i] = nd(gen);
+    s.sh_nodes.assign((size_t)s.noise_repeats * s.nodes, 0);
+    s.sh_ts.assign((size_t)s.noise_repeats * s.noise_bw_it, 0);
+    std::vector<int> idx(s.nodes);
+    std::iota(idx.begin(), idx.end(), 0);
+    for (int r = 0; r < s.noise_repeats; r++) {
+        std::shuffle(idx.begin(), idx.end(), gen);
+        std::copy(idx.begin(), idx.end(), s.sh_nodes.begin() + (size_t)r * s.nodes);
+    }
+    std::vector<int> tix(s.noise_bw_it);
+    std::iota(tix.begin(), tix.end(), 0);
+    for (int r = 0; r < s.noise_repeats; r++) {
+        std::shuffle(tix.begin(), tix.end(), gen);
+        std::copy(tix.begin(), tix.end(), s.sh_ts.begin() + (size_t)r * s.noise_bw_it);
+    }
+    s.d_noise.ensure(sizeof(double) * s.noise_size);
+    check(hipMemcpy(s.d_noise.p, s.noise.data(), sizeof(double) * s.noise_size, hipMemcpyHostToDevice), "noise H2D");
+    s.d_shn.ensure(sizeof(int) * s.sh_nodes.size());
+    check(hipMemcpy(s.d_shn.p, s.sh_nodes.data(), sizeof(int) * s.sh_nodes.size(), hipMemcpyHostToDevice), "shuffle H2D");
+    s.d_sht.ensure(sizeof(int) * s.sh_ts.size());
+    check(hipMemcpy(s.d_sht.p, s.sh_ts.data(), sizeof(int) * s.sh_ts.size(), hipMemcpyHostToDevice), "shuffle H2D");
+    s.last_seed = s.sim_seed; s.last_time_steps = s.time_steps; s.last_nodes = s.nodes;
+    s.last_noise_ts = s.conf.noise_time_steps; s.last_n_noise = n_noise;
+    s.noise_ready = true;
+}
+
+void upload_tables(Session& s) {
+    auto up = [](DevBuf& b, const std::vector<int>& a, const std::vector<int>& c) {
+        b.ensure(sizeof(int) * std::max<size_t>(1, a.size() * 2));
+        if (!a.empty()) {
+            check(hipMemcpy(b.p, a.data(), sizeof(int) * a.size(), hipMemcpyHostToDevice), "table H2D");
+            check(hipMemcpy(b.as<int>() + a.size(), c.data(), sizeof(int) * c.size(), hipMemcpyHostToDevice), "table H2D");
+        }
+    };
+    up(s.d_pairs, s.pairs_i, s.pairs_j);
+    up(s.d_wins, s.w_starts, s.w_ends);
+    up(s.d_wpairs, s.wpairs_i, s.wpairs_j);
+}
+
+// ---------------------------------------------------------------- analytical FIC
+// Brent's (1973) bracketing zero finder, iterated the way the reference
+// drives gsl_root_fsolver_brent (rww_fic.cpp:62-102): one step per iterate,
+// stop once the bracket satisfies |hi-lo| < 1e-3*min(|lo|,|hi|) (same sign),
+// at most 100 iterates; -1 when the bracket does not straddle a root or the
+// iteration does not converge.
+double phi_I(const RWW::C& c, double II) {
+    return ((c.a_I * II) - c.b_I) / (1 - std::exp(-1 * c.d_I * ((c.a_I * II) - c.b_I)));
+}
+
+double inh_fixed_point(const RWW::C& c, double w_EI, double x) {
+    const double I0_I = c.I_0 * c.w_I;
+    return I0_I + w_EI * c.S_E_ss - c.w_II * c.gamma_I_s * c.tau_I_s * phi_I(c, x) - x;
+}
+
+double fic_root(const RWW::C& c, double w_EI) {
+    const double eps = 2.2204460492503131e-16;
+    double lo0 = 0.0, hi0 = 2.0;
+    double fa = inh_fixed_point(c, w_EI, lo0), fb = inh_fixed_point(c, w_EI, hi0);
+    if ((fa < 0.0 && fb < 0.0) || (fa > 0.0 && fb > 0.0)) return -1;
+    double a = lo0, b = hi0, cc = hi0, fc = fb, d = hi0 - lo0, e = hi0 - lo0;
+    for (int it = 0; it < 100; it++) {
+        bool interp3 = true;  // false when the third point coincides with a (secant step)
+        if ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) { interp3 = false; cc = a; fc = fa; d = b - a; e = b - a; }
+        if (std::fabs(fc) < std::fabs(fb)) { interp3 = false; a = b; b = cc; cc = a; fa = fb; fb = fc; fc = fa; }
+        const double tol = 0.5 * eps * std::fabs(b);
+        const double m = 0.5 * (cc - b);
+        double root, lo, hi;
+        if (fb == 0) { root = lo = hi = b; }
+        else if (std::fabs(m) <= tol) { root = b; lo = std::min(b, cc); hi = std::max(b, cc); if (b == cc) lo = hi = b; }
+        else {
+            if (std::fabs(e) < tol || std::fabs(fa) <= std::fabs(fb)) { d = m; e = m; }
+            else {
+                const double sr = fb / fa;
+                double p, q;
+                if (!interp3) { p = 2 * m * sr; q = 1 - sr; }
+                else {
+                    const double qa = fa / fc, rb = fb / fc;
+                    p = sr * (2 * m * qa * (qa - rb) - (b - a) * (rb - 1));
+                    q = (qa - 1) * (rb - 1) * (sr - 1);
+                }
+                if (p > 0) q = -q; else p = -p;
+                if (2 * p < std::min(3 * m * q - std::fabs(tol * q), std::fabs(e * q))) { e = d; d = p / q; }
+                else { d = m; e = m; }
+            }
+            a = b; fa = fb;
+            b += (std::fabs(d) > tol) ? d : (m > 0 ? +tol : -tol);
+            fb = inh_fixed_point(c, w_EI, b);
+            root = b;
+            const double other = ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) ? a : cc;
+            lo = std::min(b, other); hi = std::max(b, other);
+        }
+        const bool same_sign = (lo > 0.0 && hi > 0.0) || (lo < 0.0 && hi < 0.0);
+        const double min_abs = same_sign ? std::min(std::fabs(lo), std::fabs(hi)) : 0.0;
+        if (std::fabs(hi - lo) < 0.001 * min_abs) return root;
+    }
+    return -1;
+}
+
+// rww.cpp:73-146 + rww_fic.cpp:138-231 for all simulations, threaded over sims.
+// The root depends only on J_N, so roots are memoised per distinct J_N value.
+void analytical_fic(const Session& s, const double* SC, const int* SCi, const double* gp, double* rp,
+                    unsigned char* fic_unstable) {
+    const int N = s.nodes, S = s.N_SIMS;
+    const RWW::C& c = s.rww;
+    std::unordered_map<uint64_t, double> roots;
+    for (long long i = 0; i < (long long)S * N; i++) {
+        const double jn = rp[1LL * S * N + i];
+        uint64_t key; std::memcpy(&key, &jn, 8);
+        if (!roots.count(key)) roots[key] = fic_root(c, jn);
+    }
+    const double I0_E = c.I_0 * c.w_E;
+    auto work = [&](int s0, int s1) {
+        std::vector<double> wIE(N);
+        for (int sim = s0; sim < s1; sim++) {
+            const double G = gp[sim];
+            const double* sc = SC + (long long)SCi[sim] * N * N;
+            bool unstable = false;
+            for (int j = 0; j < N && !unstable; j++) {
+                const double jn = rp[1LL * S * N + (long long)sim * N + j];
+                const double wEE = rp[(long long)sim * N + j] * jn;
+                uint64_t key; std::memcpy(&key, &jn, 8);
+                const double II = roots.at(key);
+                if (II == -1) { unstable = true; break; }
+                const double SI = phi_I(c, II) * c.tau_I_s * c.gamma_I_s;
+                double dot = 0.0;
+                for (int k = 0; k < N; k++) {
+                    double v = (sc[(long long)k * N + j] * G) * jn;
+                    if (k == j) v += wEE;
+                    dot += v * c.S_E_ss;
+                }
+                const double w = (-1 / SI) * (c.I_E_ss - c.I_ext - I0_E - dot);
+                if (w < 0) { unstable = true; break; }
+                wIE[j] = w;
+            }
+            fic_unstable[sim] = unstable;
+            for (int j = 0; j < N; j++) rp[2LL * S * N + (long long)sim * N + j] = unstable ? 1.0 : wIE[j];
+        }
+    };
+    int nt = (int)std::max(1u, std::min(std::thread::hardware_concurrency(), 64u));
+    nt = std::min(nt, std::max(1, S / 8));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++) th.emplace_back(work, (int)((long long)S * t / nt), (int)((long long)S * (t + 1) / nt));
+    for (auto& t : th) t.join();
+}
+
+// bnm.cpp:146-216: symmetric integer delays from the lower triangle of
+// SC_dist, diagonal 1, max over pairs; 0 when there is no pair.
+void build_delays(const Session& s, const double* SC_dist, const double* v_list, std::vector<int>& delays,
+                  std::vector<int>& maxd) {
+    const int N = s.nodes;
+    delays.assign((size_t)s.N_SIMS * N * N, 0);
+    maxd.assign(s.N_SIMS, 0);
+    for (int sim = 0; sim < s.N_SIMS; sim++) {
+        const double vel = v_list[sim] * s.dt;
+        int* d = &delays[(size_t)sim * N * N];
+        int md = 0;
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j <= i; j++) {
+                if (i == j) { d[(size_t)i * N + j] = 1; continue; }
+                int cd = (int)rintf((float)(SC_dist[(long long)i * N + j] / vel));
+                cd = std::max(cd, 1);
+                d[(size_t)i * N + j] = cd; d[(size_t)j * N + i] = cd;
+                md = std::max(md, cd);
+            }
+        maxd[sim] = md;
+    }
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.f;
+    return ms;
+}
+
+void ensure_stream(Session& s) {
+    if (s.stream) return;
+    check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : s.ev) check(hipEventCreate(&e), "hipEventCreate");
+}
+
+// Run the kernels on device-resident inputs already in s.d_* (or the caller's
+// device pointers), writing device outputs.
+struct DevIO {
+    const double* SC; const int* SCi; double* gp; double* rp;
+    double *bold, *fc, *fcd, *states; unsigned char* gb; int* gi;
+};
+
+void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, const std::vector<int>* maxd) {
+    const int N = s.nodes;
+    IntegrateArgs a{};
+    a.N_SIMS = s.N_SIMS; a.nodes = N;
+    a.bw_it = s.bw_it; a.inner_it = s.inner_it; a.BOLD_TR_iters = s.BOLD_TR_iters;
+    a.states_sampling_iters = s.states_sampling_iters; a.n_states_samples_remove = s.n_states_samples_remove;
+    a.noise_bw_it = s.noise_bw_it; a.states_ts = s.conf.states_ts; a.bold_len = s.bold_len;
+    a.states_len = s.states_len; a.noise_repeats = s.noise_repeats;
+    a.do_delay = s.do_delay; a.dt = s.dt;
+    a.SC = io.SC; a.SC_indices = io.SCi; a.gp = io.gp; a.rp = io.rp;
+    a.noise = s.d_noise.as<double>(); a.sh_nodes = s.d_shn.as<int>(); a.sh_ts = s.d_sht.as<int>();
+    a.bold = io.bold; a.states = io.states; a.gbool = io.gb; a.gint = io.gi;
+    a.bw = g_bw;
+    a.bw.dt = s.bw_dt; a.bw.dt_itau = a.bw.dt * a.bw.itau;  // core.cpp:291-292
+    bool any_delay = false;
+    if (s.do_delay) {
+        int mmax = 0;
+        for (int v : *maxd) mmax = std::max(mmax, v);
+        any_delay = mmax > 0;
+        s.d_delays.ensure(sizeof(int) * delays->size());
+        check(hipMemcpyAsync(s.d_delays.p, delays->data(), sizeof(int) * delays->size(), hipMemcpyHostToDevice, s.stream), "delay H2D");
+        s.d_maxd.ensure(sizeof(int) * maxd->size());
+        check(hipMemcpyAsync(s.d_maxd.p, maxd->data(), sizeof(int) * maxd->size(), hipMemcpyHostToDevice, s.stream), "delay H2D");
+        a.hist_stride = (long long)std::max(mmax, 1) * N;
+        s.d_hist.ensure(sizeof(double) * a.hist_stride * s.N_SIMS);
+        a.delays = s.d_delays.as<int>(); a.max_delays = s.d_maxd.as<int>(); a.hist = s.d_hist.as<double>();
+    }
+    a.do_delay = any_delay;
+    const bool fic_restarts = s.model->id == MODEL_RWW && s.rww.do_fic && s.rww.max_fic_trials > 0;
+    bool use_batched = !any_delay && !fic_restarts && !s.conf.states_ts && batched_supported(s.model->id, N);
+    if (s.force_integrator == 0) use_batched = false;
+    if (s.force_integrator == 1 && !use_batched)
+        throw std::runtime_error("batched integrator does not support this configuration (delay, numerical FIC, states_ts or node count)");
+    s.last_integrator = use_batched ? 1 : 0;
+    int launches = 1;
+    check(hipEventRecord(s.ev[2], s.stream), "event");
+    if (use_batched) check(launch_integrate_batched(s.model->id, a, model_consts(s), s.stream, &launches), "integrate_batched launch");
+    else check(launch_integrate_general(s.model->id, a, model_consts(s), s.stream), "integrate_general launch");
+    check(hipEventRecord(s.ev[3], s.stream), "event");
+    if (s.conf.do_fc) {
+        FcArgs f{};
+        f.N_SIMS = s.N_SIMS; f.nodes = N; f.bold_len = s.bold_len; f.n_vols_remove = s.n_vols_remove;
+        f.n_pairs = s.n_pairs; f.n_windows = s.conf.do_fcd ? s.n_windows : 0;
+        f.n_window_pairs = s.conf.do_fcd ? s.n_window_pairs : 0;
+        f.exc_inter = s.conf.exc_interhemispheric;
+        f.bold = io.bold;
+        f.pairs_i = s.d_pairs.as<int>(); f.pairs_j = s.d_pairs.as<int>() + s.n_pairs;
+        f.w_starts = s.d_wins.as<int>(); f.w_ends = s.d_wins.as<int>() + s.n_windows;
+        f.wpairs_i = s.d_wpairs.as<int>(); f.wpairs_j = s.d_wpairs.as<int>() + s.n_window_pairs;
+        f.fc = io.fc; f.fcd = io.fcd;
+        f.n_slots = std::min(s.N_SIMS, 2048);
+        s.d_wfc.ensure(sizeof(double) * std::max<size_t>(1, (size_t)f.n_slots * f.n_windows * f.n_pairs));
+        f.wfc_scratch = s.d_wfc.as<double>();
+        check(launch_fc_fcd(f, s.stream), "fc_fcd launch");
+    }
+    check(hipEventRecord(s.ev[4], s.stream), "event");
+    s.timings[5] = launches;
+}
+
+void require_session() {
+    if (!g_s.initialized || !g_s.model) throw std::runtime_error("cubnm_prepare must be called before cubnm_run");
+}
+
+}  // namespace
+
+void set_last_error(const std::string& e) { g_err = e; }
+
+}  // namespace cubnm
+
+using namespace cubnm;
+
+extern "C" {
+
+const char* cubnm_last_error(void) { return g_err.c_str(); }
+
+int cubnm_init(void) {
+    bw_defaults(g_bw);
+    g_bw_ready = true;
+    return 0;
+}
+
+int cubnm_set_const(const char* key, double value) {
+    if (!g_bw_ready) cubnm_init();
+    if (std::strcmp(key, "k1") == 0) g_bw.k1 = value;
+    else if (std::strcmp(key, "k2") == 0) g_bw.k2 = value;
+    else if (std::strcmp(key, "k3") == 0) g_bw.k3 = value;
+    else if (std::strcmp(key, "V_0") == 0) g_bw.V_0 = value;
+    return 0;
+}
+
+int cubnm_force_integrator(int force) { g_s.force_integrator = force; return 0; }
+int cubnm_last_integrator(void) { return g_s.last_integrator; }
+
+int cubnm_last_timings(double* out6) {
+    for (int i = 0; i < 6; i++) out6[i] = g_s.timings[i];
+    return 0;
+}
+
+int cubnm_prepare(const char* model_name, int n_config, const char* const* keys, const char* const* values,
+                  int ext_out, int states_ts, int noise_out, int do_delay, int force_reinit, int use_cpu,
+                  int N_SIMS, int nodes, int N_SCs, int time_steps, int BOLD_TR, int states_sampling,
+                  int sim_seed, double dt, double bw_dt, cubnm_dims* dims) {
+    try {
+        if (!g_bw_ready) cubnm_init();
+        if (use_cpu)
+            throw std::runtime_error("use_cpu is not supported: this library is the GPU simulation core "
+                                     "(the CPU restatement is test infrastructure only)");
+        const ModelInfo* m = find_model(model_name);
+        if (!m) throw std::runtime_error(std::string("Model ") + model_name + " not found");
+        if (N_SIMS <= 0 || nodes <= 0 || N_SCs <= 0) throw std::runtime_error("N_SIMS, nodes and N_SCs must be positive");
+        if (nodes > 1024) throw std::runtime_error("nodes > 1024 is not supported");
+        if (dt <= 0 || bw_dt <= 0 || BOLD_TR <= 0 || states_sampling <= 0 || time_steps <= 0)
+            throw std::runtime_error("durations and steps must be positive");
+        Session& s = g_s;
+        const double t0 = now_s();
+        const bool new_model = (s.model != m) || !s.initialized;
+        if (new_model) { s.noise_ready = false; }
+        s.model = m;
+        s.N_SIMS = N_SIMS; s.nodes = nodes; s.N_SCs = N_SCs; s.BOLD_TR = BOLD_TR;
+        s.states_sampling = states_sampling; s.time_steps = time_steps; s.do_delay = do_delay != 0;
+        s.sim_seed = sim_seed; s.dt = dt; s.bw_dt = bw_dt / 1000.0; s.noise_out = noise_out != 0;
+        std::vector<std::pair<std::string, std::string>> kv;
+        for (int i = 0; i < n_config; i++) kv.emplace_back(keys[i], values[i]);
+        // rWW config defaults are reset on every call (rww.hpp:61-69)
+        RWW::init_constants(s.rww, dt);
+        apply_config(s, kv);
+        s.conf.ext_out = ext_out != 0;
+        s.conf.states_ts = states_ts != 0;
+        init_model_constants(s);
+        derive_shape(s);
+        ensure_stream(s);
+        ensure_noise(s, force_reinit != 0);
+        upload_tables(s);
+        s.initialized = true;
+        dims->bold_len = s.bold_len; dims->n_pairs = s.n_pairs; dims->n_windows = s.n_windows;
+        dims->n_window_pairs = s.n_window_pairs; dims->states_len = s.states_len;
+        dims->n_state_vars = m->n_state; dims->n_global_params = m->n_global; dims->n_regional_params = m->n_regional;
+        dims->n_global_out_bool = m->n_gb; dims->n_global_out_int = m->n_gi;
+        dims->noise_bw_it = s.noise_bw_it; dims->noise_repeats = s.noise_repeats; dims->bw_it = s.bw_it;
+        dims->inner_it = s.inner_it; dims->n_vols_remove = s.n_vols_remove; dims->corr_len = s.corr_len;
+        dims->do_fc = s.conf.do_fc; dims->do_fcd = s.conf.do_fc && s.conf.do_fcd;
+        dims->modifies_params = s.modifies_params; dims->N_SIMS = N_SIMS; dims->nodes = nodes;
+        dims->noise_size = s.noise_size;
+        dims->init_time = now_s() - t0;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int cubnm_run(const double* SC, const int* SC_indices, const double* SC_dist, double* gp, double* rp,
+              const double* v_list, cubnm_outputs* out) {
+    try {
+        require_session();
+        Session& s = g_s;
+        const double t_start = now_s();
+        const int N = s.nodes, S = s.N_SIMS;
+        const ModelInfo* m = s.model;
+        for (int i = 0; i < S; i++)
+            if (SC_indices[i] < 0 || SC_indices[i] >= s.N_SCs) throw std::runtime_error("SC_indices out of range");
+        // host parameter preparation (rWW analytical FIC)
+        std::vector<unsigned char> gb((size_t)std::max(1, m->n_gb) * S, 0);
+        const double tp0 = now_s();
+        if (m->id == MODEL_RWW && s.rww.do_fic) analytical_fic(s, SC, SC_indices, gp, rp, gb.data());
+        std::vector<int> delays, maxd;
+        if (s.do_delay) build_delays(s, SC_dist, v_list, delays, maxd);
+        s.timings[0] = (now_s() - tp0) * 1e3;
+        // H2D
+        const size_t scb = sizeof(double) * (size_t)s.N_SCs * N * N;
+        const size_t gpb = sizeof(double) * (size_t)m->n_global * S;
+        const size_t rpb = sizeof(double) * (size_t)m->n_regional * S * N;
+        const long long st_stride = s.conf.states_ts ? (long long)s.states_len * N : N;
+        const size_t stb = sizeof(double) * (size_t)m->n_state * S * st_stride;
+        s.d_SC.ensure(scb); s.d_SCi.ensure(sizeof(int) * S); s.d_gp.ensure(gpb); s.d_rp.ensure(rpb);
+        s.d_bold.ensure(sizeof(double) * (size_t)S * s.bold_size);
+        s.d_fc.ensure(sizeof(double) * std::max<size_t>(1, (size_t)S * s.n_pairs));
+        s.d_fcd.ensure(sizeof(double) * std::max<size_t>(1, (size_t)S * s.n_window_pairs));
+        s.d_states.ensure(stb);
+        s.d_gb.ensure(std::max<size_t>(1, gb.size()));
+        s.d_gi.ensure(sizeof(int) * std::max<size_t>(1, (size_t)m->n_gi * S));
+        check(hipEventRecord(s.ev[0], s.stream), "event");
+        check(hipMemcpyAsync(s.d_SC.p, SC, scb, hipMemcpyHostToDevice, s.stream), "SC H2D");
+        check(hipMemcpyAsync(s.d_SCi.p, SC_indices, sizeof(int) * S, hipMemcpyHostToDevice, s.stream), "SC_indices H2D");
+        check(hipMemcpyAsync(s.d_gp.p, gp, gpb, hipMemcpyHostToDevice, s.stream), "params H2D");
+        check(hipMemcpyAsync(s.d_rp.p, rp, rpb, hipMemcpyHostToDevice, s.stream), "params H2D");
+        check(hipMemcpyAsync(s.d_gb.p, gb.data(), gb.size(), hipMemcpyHostToDevice, s.stream), "bools H2D");
+        check(hipMemsetAsync(s.d_gi.p, 0, sizeof(int) * std::max<size_t>(1, (size_t)m->n_gi * S), s.stream), "memset");
+        check(hipEventRecord(s.ev[1], s.stream), "event");
+        DevIO io{s.d_SC.as<double>(), s.d_SCi.as<int>(), s.d_gp.as<double>(), s.d_rp.as<double>(),
+                 s.d_bold.as<double>(), s.d_fc.as<double>(), s.d_fcd.as<double>(), s.d_states.as<double>(),
+                 s.d_gb.as<unsigned char>(), s.d_gi.as<int>()};
+        run_kernels(s, io, s.do_delay ? &delays : nullptr, s.do_delay ? &maxd : nullptr);
+        // D2H
+        check(hipMemcpyAsync(out->sim_bold, s.d_bold.p, sizeof(double) * (size_t)S * s.bold_size, hipMemcpyDeviceToHost, s.stream), "BOLD D2H");
+        if (s.conf.do_fc && out->sim_fc_trils)
+            check(hipMemcpyAsync(out->sim_fc_trils, s.d_fc.p, sizeof(double) * (size_t)S * s.n_pairs, hipMemcpyDeviceToHost, s.stream), "FC D2H");
+        if (s.conf.do_fc && s.conf.do_fcd && out->sim_fcd_trils)
+            check(hipMemcpyAsync(out->sim_fcd_trils, s.d_fcd.p, sizeof(double) * (size_t)S * s.n_window_pairs, hipMemcpyDeviceToHost, s.stream), "FCD D2H");
+        if (out->sim_states) check(hipMemcpyAsync(out->sim_states, s.d_states.p, stb, hipMemcpyDeviceToHost, s.stream), "states D2H");
+        if (out->global_bools && m->n_gb > 0)
+            check(hipMemcpyAsync(out->global_bools, s.d_gb.p, (size_t)m->n_gb * S, hipMemcpyDeviceToHost, s.stream), "bools D2H");
+        if (out->global_ints && m->n_gi > 0)
+            check(hipMemcpyAsync(out->global_ints, s.d_gi.p, sizeof(int) * (size_t)m->n_gi * S, hipMemcpyDeviceToHost, s.stream), "ints D2H");
+        if (s.modifies_params) {
+            check(hipMemcpyAsync(gp, s.d_gp.p, gpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
+            check(hipMemcpyAsync(rp, s.d_rp.p, rpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
+        }
+        check(hipEventRecord(s.ev[5], s.stream), "event");
+        check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+        s.timings[1] = ev_ms(s.ev[0], s.ev[1]);
+        s.timings[2] = ev_ms(s.ev[2], s.ev[3]);
+        s.timings[3] = ev_ms(s.ev[3], s.ev[4]);
+        s.timings[4] = ev_ms(s.ev[4], s.ev[5]);
+        if (s.noise_out) {
+            if (out->noise) std::copy(s.noise.begin(), s.noise.end(), out->noise);
+            if (out->shuffled_nodes) std::copy(s.sh_nodes.begin(), s.sh_nodes.end(), out->shuffled_nodes);
+            if (out->shuffled_ts) std::copy(s.sh_ts.begin(), s.sh_ts.end(), out->shuffled_ts);
+        }
+        out->run_time = now_s() - t_start;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int cubnm_run_device(const double* d_SC, const int* d_SCi, const double* d_SC_dist, double* d_gp, double* d_rp,
+                     const double* d_v, cubnm_outputs* o) {
+    try {
+        require_session();
+        Session& s = g_s;
+        const double t_start = now_s();
+        const ModelInfo* m = s.model;
+        const int S = s.N_SIMS, N = s.nodes;
+        std::vector<int> delays, maxd;
+        s.timings[0] = 0;
+        const long long st_stride = s.conf.states_ts ? (long long)s.states_len * N : N;
+        if (!o->sim_states) s.d_states.ensure(sizeof(double) * (size_t)m->n_state * S * st_stride);
+        if (!o->global_bools) s.d_gb.ensure(std::max<size_t>(1, (size_t)m->n_gb * S));
+        if (!o->global_ints) s.d_gi.ensure(sizeof(int) * std::max<size_t>(1, (size_t)m->n_gi * S));
+        if (!o->sim_fc_trils) s.d_fc.ensure(sizeof(double) * std::max<size_t>(1, (size_t)S * s.n_pairs));
+        if (!o->sim_fcd_trils) s.d_fcd.ensure(sizeof(double) * std::max<size_t>(1, (size_t)S * s.n_window_pairs));
+        unsigned char* d_gb = o->global_bools ? o->global_bools : s.d_gb.as<unsigned char>();
+        int* d_gi = o->global_ints ? o->global_ints : s.d_gi.as<int>();
+        if (m->n_gb > 0) check(hipMemsetAsync(d_gb, 0, (size_t)m->n_gb * S, s.stream), "memset");
+        if (m->n_gi > 0) check(hipMemsetAsync(d_gi, 0, sizeof(int) * (size_t)m->n_gi * S, s.stream), "memset");
+        auto d2h = [&](void* dst, const void* src, size_t n) {
+            check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s.stream), "D2H");
+        };
+        if (m->id == MODEL_RWW && s.rww.do_fic) {
+            // FIC is host arithmetic: stage the parameters through the host.
+            const double tp0 = now_s();
+            std::vector<double> SC((size_t)s.N_SCs * N * N), gp((size_t)m->n_global * S), rp((size_t)m->n_regional * S * N);
+            std::vector<int> SCi(S);
+            d2h(SC.data(), d_SC, sizeof(double) * SC.size());
+            d2h(SCi.data(), d_SCi, sizeof(int) * S);
+            d2h(gp.data(), d_gp, sizeof(double) * gp.size());
+            d2h(rp.data(), d_rp, sizeof(double) * rp.size());
+            check(hipStreamSynchronize(s.stream), "sync");
+            std::vector<unsigned char> gb((size_t)m->n_gb * S, 0);
+            analytical_fic(s, SC.data(), SCi.data(), gp.data(), rp.data(), gb.data());
+            check(hipMemcpyAsync(d_rp, rp.data(), sizeof(double) * rp.size(), hipMemcpyHostToDevice, s.stream), "H2D");
+            check(hipMemcpyAsync(d_gb, gb.data(), gb.size(), hipMemcpyHostToDevice, s.stream), "H2D");
+            check(hipStreamSynchronize(s.stream), "sync");
+            s.timings[0] = (now_s() - tp0) * 1e3;
+        }
+        if (s.do_delay) {
+            std::vector<double> dist((size_t)N * N), v(S);
+            d2h(dist.data(), d_SC_dist, sizeof(double) * dist.size());
+            d2h(v.data(), d_v, sizeof(double) * S);
+            check(hipStreamSynchronize(s.stream), "sync");
+            build_delays(s, dist.data(), v.data(), delays, maxd);
+        }
+        DevIO io{d_SC, d_SCi, d_gp, d_rp, o->sim_bold,
+                 o->sim_fc_trils ? o->sim_fc_trils : s.d_fc.as<double>(),
+                 o->sim_fcd_trils ? o->sim_fcd_trils : s.d_fcd.as<double>(),
+                 o->sim_states ? o->sim_states : s.d_states.as<double>(),
+                 d_gb, d_gi};
+        check(hipEventRecord(s.ev[0], s.stream), "event");
+        check(hipEventRecord(s.ev[1], s.stream), "event");
+        run_kernels(s, io, s.do_delay ? &delays : nullptr, s.do_delay ? &maxd : nullptr);
+        check(hipEventRecord(s.ev[5], s.stream), "event");
+        check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+        s.timings[1] = 0;
+        s.timings[2] = ev_ms(s.ev[2], s.ev[3]);
+        s.timings[3] = ev_ms(s.ev[3], s.ev[4]);
+        s.timings[4] = 0;
+        o->run_time = now_s() - t_start;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+/*
+ * cubnm_amd — C ABI of the MI355X simulation core.
+ *
+ * The reference exposes its core as the CPython extension `cubnm.core`
+ * (src/ext/core.cpp:507-626) with three functions: run_simulations, init and
+ * set_const.  This header is the plain-C form of that boundary: pointers,
+ * sizes and strings only.  `cubnm_amd.core` (a CPython module built on top of
+ * this library) re-exports the exact Python signatures, and ctypes / cgo /
+ * any FFI can bind these symbols directly.
+ *
+ * Array layouts are the reference's (core.cpp:194-493, sim/base.py:705-750):
+ *   SC               [N_SCs][nodes*nodes], rows = sources, cols = targets
+ *   SC_indices       [N_SIMS] int32
+ *   SC_dist          [nodes*nodes]
+ *   global_params    [n_global_params][N_SIMS]
+ *   regional_params  [n_regional_params][N_SIMS*nodes]
+ *   v_list           [N_SIMS]
+ *   sim_bold         [N_SIMS][bold_len*nodes]
+ *   sim_fc_trils     [N_SIMS][n_pairs]          lower triangle, row-major
+ *   sim_fcd_trils    [N_SIMS][n_window_pairs]
+ *   sim_states       [n_state_vars][N_SIMS][nodes (or nodes*states_len)]
+ *   global_bools     [n_global_out_bool][N_SIMS] (uint8)
+ *   global_ints      [n_global_out_int][N_SIMS]
+ *   noise            [noise_size]; shuffled_nodes [noise_repeats][nodes];
+ *   shuffled_ts      [noise_repeats][noise_bw_it]
+ *
+ * Every function returns 0 on success and a negative value on error; the
+ * message is then available from cubnm_last_error().  Not re-entrant: the
+ * session is process-global, as in the reference.
+ */
+#ifndef CUBNM_AMD_H
+#define CUBNM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cubnm_dims {
+    int bold_len, n_pairs, n_windows, n_window_pairs, states_len, n_state_vars;
+    int n_global_params, n_regional_params, n_global_out_bool, n_global_out_int;
+    int noise_bw_it, noise_repeats, bw_it, inner_it, n_vols_remove, corr_len;
+    int do_fc, do_fcd, modifies_params, N_SIMS, nodes;
+    long long noise_size;
+    double init_time; /* seconds spent initialising the session in cubnm_prepare */
+} cubnm_dims;
+
+typedef struct cubnm_outputs {
+    double* sim_bold;             /* required */
+    double* sim_fc_trils;         /* may be NULL when FC is off or not wanted */
+    double* sim_fcd_trils;        /* may be NULL */
+    double* sim_states;           /* may be NULL (ext_out off) */
+    unsigned char* global_bools;  /* may be NULL */
+    int* global_ints;             /* may be NULL */
+    double* noise;                /* may be NULL (noise_out off) */
+    int* shuffled_nodes;          /* may be NULL */
+    int* shuffled_ts;             /* may be NULL */
+    double run_time;              /* out: seconds of the run phase */
+} cubnm_outputs;
+
+/* init(): initialise the Balloon-Windkessel constants (core.cpp:147-155). */
+int cubnm_init(void);
+
+/* set_const(key, value): set BW k1, k2, k3 or V_0 and refresh the derived
+ * constants (core.cpp:157-192).  Unknown keys are ignored, as upstream. */
+int cubnm_set_const(const char* key, double value);
+
+/* First half of run_simulations (core.cpp:194-386): select or rebuild the
+ * model session, apply the string config (base.hpp:271-297, rww.cpp:54-71),
+ * (re)generate the noise segment when seed/duration/nodes/segment changed or
+ * force_reinit, and report the output dimensions the caller must allocate.
+ * use_cpu != 0 is rejected: this library is the GPU core. */
+int cubnm_prepare(const char* model_name, int n_config, const char* const* config_keys,
+                  const char* const* config_values, int ext_out, int states_ts, int noise_out,
+                  int do_delay, int force_reinit, int use_cpu, int N_SIMS, int nodes, int N_SCs,
+                  int time_steps, int BOLD_TR, int states_sampling, int sim_seed, double dt,
+                  double bw_dt, cubnm_dims* dims);
+
+/* Second half of run_simulations (core.cpp:388-450) on host buffers: model
+ * parameter preparation (rWW analytical FIC writes wIE into regional_params
+ * in place, rww.cpp:73-146), the simulations, FC/FCD, and write-back of
+ * parameters modified during the run (numerical FIC). */
+int cubnm_run(const double* SC, const int* SC_indices, const double* SC_dist,
+              double* global_params, double* regional_params, const double* v_list,
+              cubnm_outputs* out);
+
+/* Same as cubnm_run, but every array pointer (inputs and outputs) is device
+ * memory already resident in HBM (hipMalloc'd or a torch CUDA tensor).  For
+ * callers that keep data on the GPU; no reference counterpart. The noise
+ * outputs are ignored (host-generated, see cubnm_run). */
+int cubnm_run_device(const double* d_SC, const int* d_SC_indices, const double* d_SC_dist,
+                     double* d_global_params, double* d_regional_params, const double* d_v_list,
+                     cubnm_outputs* d_out);
+
+/* Timings of the last run (milliseconds): [0] parameter prep on host,
+ * [1] host->device copies, [2] integration kernel, [3] FC/FCD kernel,
+ * [4] device->host copies; [5] integration kernel launches. */
+int cubnm_last_timings(double* out6);
+
+/* Which integrator the next/last run uses: 0 = general (workgroup per
+ * simulation), 1 = batched MFMA.  Set force to -1 for automatic choice. */
+int cubnm_force_integrator(int force);
+int cubnm_last_integrator(void);
+
+const char* cubnm_last_error(void);
+
+/* Goodness of fit on the GPU (utils.py:306-507 fc_corr_device and
+ * fcd_ks_device, sim/base.py:1084-1207): Pearson r between each simulated FC
+ * tril and the empirical one, and the two-sample Kolmogorov-Smirnov
+ * statistic between each simulated FCD tril and the empirical FCD tril.
+ * Host buffers; NaN inputs yield NaN (callers map NaN to the worst score). */
+int cubnm_gof_fc_corr(const double* sim_fc, int n_sims, int n_pairs, const double* emp_fc, double* out);
+int cubnm_gof_fcd_ks(const double* sim_fcd, int n_sims, int n_sim_values, const double* emp_fcd,
+                     int n_emp_values, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUBNM_AMD_H */

@@ -1,0 +1,125 @@
+"""HIP core vs CPU oracle on identical inputs and identical host noise.
+
+Tolerances (FP64 throughout): BOLD norm-wise relative error
+max|Δ|/max|BOLD| <= 1e-9 per simulation; FC and FCD absolute error <= 1e-9;
+time-mean states relative <= 1e-9; FIC-written wIE relative <= 1e-12 (the
+analytical FIC is host arithmetic in both).  Kuramoto is chaotic: its phase
+trajectories separate from the oracle at a rate set by the largest Lyapunov
+exponent, so it is checked over a short horizon only (tolerance stated there).
+"""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+@pytest.fixture(scope="module")
+def core():
+    import cubnm_amd
+    from cubnm_amd import _core
+    _core.force_integrator(-1)
+    return _core
+
+
+def _compare(out, ref, model, tol=TOL, check_states=True):
+    rel = cases.bold_rel_err(out["sim_bold"], ref["sim_bold"])
+    assert rel.max() <= tol, f"BOLD rel err {rel.max():.3e}"
+    if "sim_fc_trils" in ref:
+        e = np.abs(out["sim_fc_trils"] - ref["sim_fc_trils"]).max()
+        assert e <= tol, f"FC abs err {e:.3e}"
+    if "sim_fcd_trils" in ref:
+        e = np.abs(out["sim_fcd_trils"] - ref["sim_fcd_trils"]).max()
+        assert e <= tol, f"FCD abs err {e:.3e}"
+    if check_states:
+        s, r = out["_sim_states"], ref["_sim_states"]
+        scale = np.maximum(np.abs(r).max(axis=(1, 2), keepdims=True), 1e-300)
+        e = (np.abs(s - r) / scale).max()
+        assert e <= tol, f"states rel err {e:.3e}"
+    np.testing.assert_array_equal(out["_global_bools"], ref["_global_bools"])
+    np.testing.assert_array_equal(out["_global_ints"], ref["_global_ints"])
+
+
+def _run_both(core, args):
+    a1, a2 = cases.copy_args(args), cases.copy_args(args)
+    out = core.run_simulations(*a1)
+    ref = cases.run_oracle(a2)
+    return out, ref, a1, a2
+
+
+@pytest.mark.parametrize("model", ["rWW", "JR", "WC"])
+def test_small_network_all_models(core, model):
+    N, S = 10, 3
+    sc = cases.synthetic_sc(N, seed=1)
+    gp, rp = cases.params(model, S, N, G=[0.5, 1.0, 2.0])
+    cfg = cases.model_config(model, TR=1.0, window_size=4, window_step=2, bold_remove_s=2)
+    args = cases.sim_args(model, sc, gp, rp, cfg, S, duration_s=12, noise_out=True, force_reinit=True)
+    out, ref, a1, a2 = _run_both(core, args)
+    _compare(out, ref, model)
+    np.testing.assert_array_equal(out["_noise"], ref["_noise"])
+    np.testing.assert_array_equal(out["_shuffled_nodes"], ref["_shuffled_nodes"])
+    np.testing.assert_array_equal(out["_shuffled_ts"], ref["_shuffled_ts"])
+    np.testing.assert_allclose(a1[5], a2[5], rtol=1e-12, atol=0)
+
+
+def test_kuramoto_short_horizon(core):
+    # chaotic model: 4 s of simulated time keeps trajectories within 1e-7
+    N, S = 10, 2
+    sc = cases.synthetic_sc(N, seed=2)
+    gp, rp = cases.params("Kuramoto", S, N, G=[0.5, 1.0])
+    cfg = cases.model_config("Kuramoto", TR=0.5, window_size=1, window_step=1, bold_remove_s=1, do_fcd=False)
+    args = cases.sim_args("Kuramoto", sc, gp, rp, cfg, S, duration_s=4, TR_s=0.5)
+    out, ref, _, _ = _run_both(core, args)
+    _compare(out, ref, "Kuramoto", tol=1e-7)
+
+
+@pytest.mark.parametrize("fic", [0, 1, 2])
+def test_rww_fic_modes(core, fic):
+    N, S = 16, 4
+    sc = cases.synthetic_sc(N, seed=3) * 0.05
+    gp, rp = cases.params("rWW", S, N, G=[0.5, 1.0, 1.5, 2.0], do_fic=fic > 0)
+    cfg = cases.model_config("rWW", TR=1.0, window_size=4, window_step=2, bold_remove_s=10,
+                             do_fic=fic > 0, max_fic_trials=5 if fic == 2 else 0)
+    args = cases.sim_args("rWW", sc, gp, rp, cfg, S, duration_s=20)
+    out, ref, a1, a2 = _run_both(core, args)
+    _compare(out, ref, "rWW")
+    np.testing.assert_allclose(a1[5], a2[5], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("model", ["rWW", "JR"])
+def test_conduction_delay(core, model):
+    N, S = 12, 2
+    sc = cases.synthetic_sc(N, seed=4)
+    dist = cases.synthetic_sc(N, seed=5) * 60.0
+    gp, rp = cases.params(model, S, N, G=[0.5, 1.0])
+    cfg = cases.model_config(model, TR=1.0, window_size=4, window_step=2, bold_remove_s=2)
+    args = cases.sim_args(model, sc, gp, rp, cfg, S, duration_s=10, dt="1.0", sc_dist=dist, v=[0.5, 2.0])
+    out, ref, _, _ = _run_both(core, args)
+    _compare(out, ref, model)
+
+
+def test_states_time_series(core):
+    N, S = 8, 2
+    sc = cases.synthetic_sc(N, seed=6)
+    gp, rp = cases.params("WC", S, N, G=[0.5, 1.0])
+    cfg = cases.model_config("WC", TR=0.5, window_size=2, window_step=1, bold_remove_s=1)
+    args = cases.sim_args("WC", sc, gp, rp, cfg, S, duration_s=6, TR_s=0.5, states_ts=True)
+    out, ref, _, _ = _run_both(core, args)
+    assert out["_sim_states"].shape == (2, S, N * 12)
+    _compare(out, ref, "WC")
+
+
+def test_c1_plain_grid_rww(core):
+    """C1: 8 sims, N=100, 60 s, G in [0.5, 3] x w_p in {0, 2}, J_N 0.15."""
+    N = 100
+    sc = cases.synthetic_sc(N, seed=0)
+    G = np.repeat(np.linspace(0.5, 3, 4), 2)
+    wp = np.tile([0.0, 2.0], 4)
+    gp, rp = cases.params("rWW", 8, N, G=G, w_p=wp[:, None], J_N=0.15, do_fic=False, wIE=1.0)
+    cfg = cases.model_config("rWW", do_fic=False)
+    args = cases.sim_args("rWW", sc, gp, rp, cfg, 8, duration_s=60)
+    out, ref, _, _ = _run_both(core, args)
+    _compare(out, ref, "rWW")
