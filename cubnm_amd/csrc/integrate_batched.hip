@@ -1,12 +1,318 @@
-// Batched MFMA integrator (placeholder until the MFMA kernel lands).
+// Batched FP64 integrator: 16 simulations per workgroup, coupling on MFMA.
+//
+// Same simulation semantics as the reference loop (src/ext/bnm.cu:117-505,
+// src/ext/bnm.cpp:69-522) for the no-delay, no-restart case.  The MI355X
+// design replaces the reference's "one block per simulation, one thread per
+// node, N LDS broadcasts per node per step" with:
+//
+//   * a workgroup integrates a group of 16 simulations that share one SC;
+//   * the coupling input of all 16 simulations for one step is the product
+//       G^T[j][c] = sum_k SC[k][j] * x_c[k]    (c = simulation in the group)
+//     computed with v_mfma_f64_16x16x4f64: A = SC^T tiles (16 nodes x 4
+//     sources) stored once per SC in MFMA fragment order (in LDS when they
+//     fit, else streamed from L2), B = the group's coupling state (4 sources x
+//     16 simulations) read from a double-buffered LDS tile, D = 16 nodes x 16
+//     simulations;
+//   * the f64 16x16x4 D layout (lane l, register r: row 4*r + l/16, column
+//     l%16; A[i][k] in lane i + 16k, B[k][j] in lane j + 16k) is the
+//     node-to-lane map of the model step: lane l updates nodes 4r + l/16 of
+//     simulation l%16 in every 16-node tile, without moving the coupling result;
+//   * each wave owns TPW node tiles; one workgroup barrier per inner step
+//     (double buffering makes a second barrier unnecessary);
+//   * oscillator coupling sum_k SC[k][j] sin(x_k - x_j) is evaluated as
+//     cos(x_j) * (SC^T sin x)_j - sin(x_j) * (SC^T cos x)_j, i.e. two products.
+//
+// The noise element of (step, node) is shared by all simulations of the
+// group (one seed), so its loads are broadcasts.  BOLD, states and FIC
+// outputs use the reference layouts.
 #include "engine.hpp"
 
 namespace cubnm {
 
-bool batched_supported(int, int) { return false; }
+typedef double v4d __attribute__((ext_vector_type(4)));
 
-hipError_t launch_integrate_batched(int, const IntegrateArgs&, const void*, hipStream_t, int*) {
-    return hipErrorNotSupported;
+struct BatchedArgs {
+    IntegrateArgs base;
+    const int* group_sims;  // [n_groups][16], -1 = padding lane
+    const double* afrag;    // [N_SCs][NT][KT][64] SC^T in MFMA A-fragment order
+    int n_groups, KT, NT;
+};
+
+namespace {
+
+template <class M>
+__device__ __forceinline__ void publish(double* xs, int kn, int j, int c, double v, double* sn, double* cs) {
+    if constexpr (M::is_osc) {
+        double sv, cv;
+        sincos(v, &sv, &cv);
+        *sn = sv; *cs = cv;
+        xs[j * 16 + c] = sv;
+        xs[kn * 16 + j * 16 + c] = cv;
+    } else {
+        xs[j * 16 + c] = v;
+    }
+}
+
+}  // namespace
+
+// A_LDS: A fragments staged in LDS (small N) or read from global memory.
+template <class M, int TPW, int MAXW, bool A_LDS>
+__global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArgs ba, typename M::C mc) {
+    constexpr int NS = 4 * TPW;                 // node slots per lane
+    constexpr int NX = M::is_osc ? 2 : 1;        // B operands per step
+    const IntegrateArgs& a = ba.base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, q = lane >> 4;
+    const int N = a.nodes, KT = ba.KT, KN = 4 * KT, NT = ba.NT;
+    const int buf_stride = NX * KN * 16;        // doubles per X buffer
+    extern __shared__ double lds[];
+    double* xs = lds;                            // [2][NX][KN][16]
+    double* as = lds + 2 * buf_stride;           // [NT][KT][64] when A_LDS
+    int loaded_sc = -1;
+
+    for (int g = blockIdx.x; g < ba.n_groups; g += gridDim.x) {
+        const int* gs = ba.group_sims + 16 * g;
+        const int sim0 = gs[0];
+        const int sim = gs[c];
+        const bool sim_ok = sim >= 0;
+        const int simr = sim_ok ? sim : sim0;
+        const int sc_idx = a.SC_indices[sim0];
+        const double* afrag = ba.afrag + (long long)sc_idx * NT * KT * 64;
+        if (A_LDS && sc_idx != loaded_sc) {
+            for (int i = threadIdx.x; i < NT * KT * 64; i += blockDim.x) as[i] = afrag[i];
+            loaded_sc = sc_idx;
+        }
+        const double* abase = A_LDS ? as : afrag;
+
+        double gp[M::n_global];
+#pragma unroll
+        for (int i = 0; i < M::n_global; i++) gp[i] = a.gp[(long long)i * a.N_SIMS + simr];
+
+        double s[NS][M::n_state], rp[NS][M::n_regional];
+        double bx[NS], bf[NS], bn[NS], bq[NS], sn[NS], cs[NS];
+        int shj[NS];
+        // zero both X buffers (padding rows stay finite: A is zero there)
+        for (int i = threadIdx.x; i < 2 * buf_stride; i += blockDim.x) xs[i] = 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int sl = 0; sl < NS; sl++) {
+            const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
+            const int jr = j < N ? j : 0;
+#pragma unroll
+            for (int i = 0; i < M::n_regional; i++)
+                rp[sl][i] = a.rp[(long long)i * a.N_SIMS * N + (long long)simr * N + jr];
+#pragma unroll
+            for (int i = 0; i < M::n_state; i++) s[sl][i] = 0.0;
+            M::init(mc, s[sl], rp[sl]);
+            bx[sl] = 0.0; bf[sl] = 1.0; bn[sl] = 1.0; bq[sl] = 1.0;
+            sn[sl] = 0.0; cs[sl] = 1.0;
+            shj[sl] = a.sh_nodes[jr];
+            if (j < N) publish<M>(xs, KN, j, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
+            if (j < N && sim_ok)
+#pragma unroll
+                for (int i = 0; i < M::n_state; i++) a.states[(long long)i * a.N_SIMS * N + (long long)sim * N + j] = 0.0;
+        }
+        __syncthreads();
+
+        double* bold = a.bold + (long long)simr * a.bold_len * N;
+        int buf = 0, bw_i = 0, bold_i = 0, states_i = 0, rep = 0;
+        while (bw_i < a.bw_it) {
+            const int sh_t = a.sh_ts[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it];
+            // states are summed from the last inner step of a sampling bw step
+            const bool sample = ((bw_i + 1) % a.states_sampling_iters == 0) &&
+                                (states_i >= a.n_states_samples_remove) && sim_ok;
+            for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
+                const double* xb = xs + buf * buf_stride;
+                v4d acc[TPW][NX];
+#pragma unroll
+                for (int t = 0; t < TPW; t++)
+#pragma unroll
+                    for (int x = 0; x < NX; x++) acc[t][x] = v4d{0.0, 0.0, 0.0, 0.0};
+                const double* ab = abase + (long long)(wave * TPW) * KT * 64 + lane;
+#pragma unroll 5
+                for (int kt = 0; kt < KT; kt++) {
+                    double af[TPW];
+#pragma unroll
+                    for (int t = 0; t < TPW; t++) af[t] = ab[(t * KT + kt) * 64];
+#pragma unroll
+                    for (int x = 0; x < NX; x++) {
+                        const double b = xb[x * KN * 16 + kt * 64 + lane];
+#pragma unroll
+                        for (int t = 0; t < TPW; t++)
+                            acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], b, acc[t][x], 0, 0, 0);
+                    }
+                }
+                double* xn = xs + (buf ^ 1) * buf_stride;
+                const long long nbase = (long long)(sh_t * a.inner_it + inner_i) * N;
+                const bool last = sample && (inner_i == a.inner_it - 1);
+#pragma unroll
+                for (int t = 0; t < TPW; t++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int sl = 4 * t + r;
+                        const int j = 16 * (wave * TPW + t) + 4 * r + q;
+                        double gin;
+                        if constexpr (M::is_osc) gin = cs[sl] * acc[t][0][r] - sn[sl] * acc[t][1][r];
+                        else gin = acc[t][0][r];
+                        const long long nidx = (nbase + shj[sl]) * M::n_noise;
+                        const double n0 = a.noise[nidx];
+                        const double n1 = (M::n_noise > 1) ? a.noise[nidx + 1] : 0.0;
+                        M::step(mc, s[sl], gp, rp[sl], gin, n0, n1);
+                        if (j < N) {
+                            publish<M>(xn, KN, j, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
+                            if (last)
+#pragma unroll
+                                for (int i = 0; i < M::n_state; i++)
+                                    a.states[(long long)i * a.N_SIMS * N + (long long)sim * N + j] += s[sl][i];
+                        }
+                    }
+                buf ^= 1;
+                __syncthreads();
+            }
+#pragma unroll
+            for (int sl = 0; sl < NS; sl++) bw_step(a.bw, s[sl][M::bold_idx], bx[sl], bf[sl], bn[sl], bq[sl]);
+            if ((bw_i + 1) % a.BOLD_TR_iters == 0) {
+                if (sim_ok && bold_i < a.bold_len)
+#pragma unroll
+                    for (int sl = 0; sl < NS; sl++) {
+                        const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
+                        if (j < N) bold[(long long)bold_i * N + j] = bold_signal(a.bw, bn[sl], bq[sl]);
+                    }
+                bold_i++;
+            }
+            if ((bw_i + 1) % a.states_sampling_iters == 0) states_i++;
+            if ((bw_i + 1) % a.noise_bw_it == 0 && bw_i + 1 < a.bw_it) {
+                rep++;
+#pragma unroll
+                for (int sl = 0; sl < NS; sl++) {
+                    const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
+                    shj[sl] = a.sh_nodes[rep * N + (j < N ? j : 0)];
+                }
+            }
+            bw_i++;
+        }
+        if (sim_ok) {
+            const int tp = states_i - a.n_states_samples_remove;
+#pragma unroll
+            for (int sl = 0; sl < NS; sl++) {
+                const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
+                if (j < N)
+#pragma unroll
+                    for (int i = 0; i < M::n_state; i++) {
+                        double* o = a.states + (long long)i * a.N_SIMS * N + (long long)sim * N + j;
+                        *o = *o / tp;
+                    }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+namespace {
+
+// SC^T in A-fragment order: frag[(jt*KT + kt)*64 + lane] = SC[(4kt + lane/16)][16jt + lane%16]
+__global__ void build_afrag_kernel(const double* SC, int n_scs, int N, int NT, int KT, double* afrag) {
+    const long long total = (long long)n_scs * NT * KT * 64;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const int lane = (int)(i % 64);
+        long long r = i / 64;
+        const int kt = (int)(r % KT); r /= KT;
+        const int jt = (int)(r % NT);
+        const int sc = (int)(r / NT);
+        const int k = 4 * kt + lane / 16, j = 16 * jt + lane % 16;
+        afrag[i] = (k < N && j < N) ? SC[(long long)sc * N * N + (long long)k * N + j] : 0.0;
+    }
+}
+
+template <class M, int TPW, int MAXW, bool A_LDS>
+hipError_t launch_one(const BatchedArgs& ba, const void* consts, int waves, hipStream_t st) {
+    if (waves > MAXW) return hipErrorInvalidConfiguration;
+    const typename M::C mc = *static_cast<const typename M::C*>(consts);
+    const int nx = M::is_osc ? 2 : 1;
+    size_t lds = sizeof(double) * 2 * nx * 4 * ba.KT * 16;
+    if (A_LDS) lds += sizeof(double) * (size_t)ba.NT * ba.KT * 64;
+    auto kern = integrate_batched_kernel<M, TPW, MAXW, A_LDS>;
+    const int threads = 64 * waves;
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    int dev = 0, n_cu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int grid = std::min(ba.n_groups, std::max(1, per_cu * n_cu));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, st, ba, mc);
+    return hipGetLastError();
+}
+
+template <class M>
+hipError_t dispatch(const BatchedArgs& ba, const void* consts, int tpw, hipStream_t st) {
+    const int waves = ba.NT / tpw;  // NT is padded to a multiple of tpw
+    const int nx = M::is_osc ? 2 : 1;
+    const size_t lds_a = sizeof(double) * ((size_t)ba.NT * ba.KT * 64 + 2 * nx * 4 * ba.KT * 16);
+    const bool a_lds = lds_a <= 150 * 1024;
+    if (tpw == 1) {
+        if (waves <= 4) return a_lds ? launch_one<M, 1, 4, true>(ba, consts, waves, st) : launch_one<M, 1, 4, false>(ba, consts, waves, st);
+        if (waves <= 8) return a_lds ? launch_one<M, 1, 8, true>(ba, consts, waves, st) : launch_one<M, 1, 8, false>(ba, consts, waves, st);
+        return hipErrorInvalidValue;
+    }
+    if (waves <= 4) return a_lds ? launch_one<M, 2, 4, true>(ba, consts, waves, st) : launch_one<M, 2, 4, false>(ba, consts, waves, st);
+    if (waves <= 8) return launch_one<M, 2, 8, false>(ba, consts, waves, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+int g_batched_tpw = 0;  // 0 = heuristic; set by the tuning entry point
+
+bool batched_supported(int model, int nodes) {
+    (void)model;
+    return nodes >= 1 && nodes <= 256;
+}
+
+// Tiles per wave and the padded tile count (waves * TPW) the fragment array
+// and the kernel use; every wave then owns exactly TPW allocated tiles.
+void batched_plan(int nodes, int* tpw, int* nt_pad) {
+    const int nt = (nodes + 15) / 16;
+    int t = g_batched_tpw;
+    if (t != 1 && t != 2) t = (nt > 8) ? 2 : 1;
+    const int waves = (nt + t - 1) / t;
+    *tpw = t;
+    *nt_pad = waves * t;
+}
+
+size_t batched_afrag_doubles(int nodes, int nt_pad, int n_scs) {
+    const int KT = (nodes + 3) / 4;
+    return (size_t)n_scs * nt_pad * KT * 64;
+}
+
+hipError_t build_afrag(const double* d_SC, int n_scs, int nodes, int nt_pad, double* d_afrag, hipStream_t s) {
+    const int KT = (nodes + 3) / 4, NT = nt_pad;
+    const long long total = (long long)n_scs * NT * KT * 64;
+    const int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(build_afrag_kernel, dim3(blocks), dim3(256), 0, s, d_SC, n_scs, nodes, NT, KT, d_afrag);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
+                                    int* launches, const int* d_group_sims, int n_groups, const double* d_afrag) {
+    BatchedArgs ba;
+    ba.base = a;
+    ba.group_sims = d_group_sims;
+    ba.afrag = d_afrag;
+    ba.n_groups = n_groups;
+    ba.KT = (a.nodes + 3) / 4;
+    int tpw, nt_pad;
+    batched_plan(a.nodes, &tpw, &nt_pad);
+    ba.NT = nt_pad;
+    *launches = 1;
+    switch (model) {
+        case MODEL_RWW: return dispatch<RWW>(ba, consts, tpw, s);
+        case MODEL_JR: return dispatch<JR>(ba, consts, tpw, s);
+        case MODEL_KURAMOTO: return dispatch<Kuramoto>(ba, consts, tpw, s);
+        case MODEL_WC: return dispatch<WC>(ba, consts, tpw, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace cubnm

@@ -29,8 +29,12 @@
 namespace cubnm {
 
 hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
-                                    int* launches);
+                                    int* launches, const int* d_group_sims, int n_groups, const double* d_afrag);
 bool batched_supported(int model, int nodes);
+void batched_plan(int nodes, int* tpw, int* nt_pad);
+size_t batched_afrag_doubles(int nodes, int nt_pad, int n_scs);
+hipError_t build_afrag(const double* d_SC, int n_scs, int nodes, int nt_pad, double* d_afrag, hipStream_t s);
+extern int g_batched_tpw;
 
 namespace {
 
@@ -122,6 +126,8 @@ struct Session {
     // device buffers
     DevBuf d_noise, d_shn, d_sht, d_pairs, d_wins, d_wpairs;
     DevBuf d_SC, d_SCi, d_gp, d_rp, d_v, d_bold, d_fc, d_fcd, d_states, d_gb, d_gi, d_delays, d_maxd, d_hist, d_wfc;
+    DevBuf d_groups, d_afrag;
+    std::vector<int> groups_h;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     double timings[6] = {0, 0, 0, 0, 0, 0};
@@ -437,7 +443,29 @@ struct DevIO {
     double *bold, *fc, *fcd, *states; unsigned char* gb; int* gi;
 };
 
-void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, const std::vector<int>* maxd) {
+// Groups of 16 simulations sharing one SC for the batched integrator: sims
+// are ordered by SC index (stable) and each SC's run is padded to a multiple
+// of 16 with -1 entries.
+int build_groups(Session& s, const int* SCi_host) {
+    std::vector<int> order(s.N_SIMS);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return SCi_host[x] < SCi_host[y]; });
+    s.groups_h.clear();
+    for (size_t i = 0; i < order.size();) {
+        size_t e = i;
+        while (e < order.size() && SCi_host[order[e]] == SCi_host[order[i]]) e++;
+        for (size_t b = i; b < e; b += 16)
+            for (size_t k = 0; k < 16; k++) s.groups_h.push_back(b + k < e ? order[b + k] : -1);
+        i = e;
+    }
+    s.d_groups.ensure(sizeof(int) * s.groups_h.size());
+    check(hipMemcpyAsync(s.d_groups.p, s.groups_h.data(), sizeof(int) * s.groups_h.size(), hipMemcpyHostToDevice,
+                         s.stream), "groups H2D");
+    return (int)(s.groups_h.size() / 16);
+}
+
+void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, const std::vector<int>* maxd,
+                 const int* SCi_host) {
     const int N = s.nodes;
     IntegrateArgs a{};
     a.N_SIMS = s.N_SIMS; a.nodes = N;
@@ -472,8 +500,19 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
         throw std::runtime_error("batched integrator does not support this configuration (delay, numerical FIC, states_ts or node count)");
     s.last_integrator = use_batched ? 1 : 0;
     int launches = 1;
+    int n_groups = 0;
+    if (use_batched) {
+        n_groups = build_groups(s, SCi_host);
+        int tpw, nt_pad;
+        batched_plan(N, &tpw, &nt_pad);
+        s.d_afrag.ensure(sizeof(double) * batched_afrag_doubles(N, nt_pad, s.N_SCs));
+        check(build_afrag(io.SC, s.N_SCs, N, nt_pad, s.d_afrag.as<double>(), s.stream), "build_afrag launch");
+    }
     check(hipEventRecord(s.ev[2], s.stream), "event");
-    if (use_batched) check(launch_integrate_batched(s.model->id, a, model_consts(s), s.stream, &launches), "integrate_batched launch");
+    if (use_batched)
+        check(launch_integrate_batched(s.model->id, a, model_consts(s), s.stream, &launches, s.d_groups.as<int>(),
+                                       n_groups, s.d_afrag.as<double>()),
+              "integrate_batched launch");
     else check(launch_integrate_general(s.model->id, a, model_consts(s), s.stream), "integrate_general launch");
     check(hipEventRecord(s.ev[3], s.stream), "event");
     if (s.conf.do_fc) {
@@ -528,6 +567,16 @@ int cubnm_set_const(const char* key, double value) {
 }
 
 int cubnm_force_integrator(int force) { g_s.force_integrator = force; return 0; }
+
+int cubnm_set_tuning(const char* key, int value) {
+    if (std::strcmp(key, "batched_tpw") == 0) {
+        if (value != 0 && value != 1 && value != 2) { g_err = "batched_tpw must be 0 (auto), 1 or 2"; return -1; }
+        g_batched_tpw = value;
+        return 0;
+    }
+    g_err = std::string("unknown tuning key ") + key;
+    return -1;
+}
 int cubnm_last_integrator(void) { return g_s.last_integrator; }
 
 int cubnm_last_timings(double* out6) {
@@ -629,7 +678,7 @@ int cubnm_run(const double* SC, const int* SC_indices, const double* SC_dist, do
         DevIO io{s.d_SC.as<double>(), s.d_SCi.as<int>(), s.d_gp.as<double>(), s.d_rp.as<double>(),
                  s.d_bold.as<double>(), s.d_fc.as<double>(), s.d_fcd.as<double>(), s.d_states.as<double>(),
                  s.d_gb.as<unsigned char>(), s.d_gi.as<int>()};
-        run_kernels(s, io, s.do_delay ? &delays : nullptr, s.do_delay ? &maxd : nullptr);
+        run_kernels(s, io, s.do_delay ? &delays : nullptr, s.do_delay ? &maxd : nullptr, SC_indices);
         // D2H
         check(hipMemcpyAsync(out->sim_bold, s.d_bold.p, sizeof(double) * (size_t)S * s.bold_size, hipMemcpyDeviceToHost, s.stream), "BOLD D2H");
         if (s.conf.do_fc && out->sim_fc_trils)
@@ -718,7 +767,12 @@ int cubnm_run_device(const double* d_SC, const int* d_SCi, const double* d_SC_di
                  d_gb, d_gi};
         check(hipEventRecord(s.ev[0], s.stream), "event");
         check(hipEventRecord(s.ev[1], s.stream), "event");
-        run_kernels(s, io, s.do_delay ? &delays : nullptr, s.do_delay ? &maxd : nullptr);
+        std::vector<int> sci_h(S);
+        d2h(sci_h.data(), d_SCi, sizeof(int) * S);
+        check(hipStreamSynchronize(s.stream), "sync");
+        for (int i = 0; i < S; i++)
+            if (sci_h[i] < 0 || sci_h[i] >= s.N_SCs) throw std::runtime_error("SC_indices out of range");
+        run_kernels(s, io, s.do_delay ? &delays : nullptr, s.do_delay ? &maxd : nullptr, sci_h.data());
         check(hipEventRecord(s.ev[5], s.stream), "event");
         check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
         s.timings[1] = 0;

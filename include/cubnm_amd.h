@@ -101,6 +101,10 @@ int cubnm_last_timings(double* out6);
 int cubnm_force_integrator(int force);
 int cubnm_last_integrator(void);
 
+/* Launch-shape knobs of the batched integrator, set by the autotuner
+ * (cubnm_amd/tune.py): "batched_tpw" = 16-node tiles per wave (0 = auto, 1, 2). */
+int cubnm_set_tuning(const char* key, int value);
+
 const char* cubnm_last_error(void);
 
 /* Goodness of fit on the GPU (utils.py:306-507 fc_corr_device and

@@ -48,7 +48,8 @@ def synthetic_sc(nodes, seed=0):
     w = rng.lognormal(0.0, 1.0, size=(nodes, nodes))
     w = (w + w.T) / 2
     np.fill_diagonal(w, 0.0)
-    return w / w.max()
+    m = w.max()
+    return w / m if m > 0 else w
 
 
 def model_config(model, TR=1.0, window_size=10, window_step=2, bold_remove_s=30,
@@ -134,7 +135,20 @@ def copy_args(args):
 
 
 def bold_rel_err(a, b):
-    """Norm-wise relative BOLD error per simulation: max|a-b| / max|b|."""
+    """Norm-wise relative BOLD error per simulation: max|a-b| / max|b|.
+    NaN in both at the same position counts as equal; NaN in one is inf."""
     a = np.asarray(a).reshape(a.shape[0], -1)
     b = np.asarray(b).reshape(b.shape[0], -1)
-    return np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)
+    d = np.abs(a - b)
+    both = np.isnan(a) & np.isnan(b)
+    d[both] = 0.0
+    d[np.isnan(a) ^ np.isnan(b)] = np.inf
+    return d.max(axis=1) / np.nanmax(np.abs(b), axis=1)
+
+
+def max_abs_err(a, b):
+    """max|a-b| with NaN-at-the-same-place treated as equal."""
+    d = np.abs(np.asarray(a) - np.asarray(b))
+    d[np.isnan(a) & np.isnan(b)] = 0.0
+    d[np.isnan(a) ^ np.isnan(b)] = np.inf
+    return d.max() if d.size else 0.0

@@ -17,27 +17,32 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-9
 
 
-@pytest.fixture(scope="module")
-def core():
-    import cubnm_amd
+@pytest.fixture(scope="module", params=["general", "batched"])
+def core(request):
+    import cubnm_amd  # noqa: F401
     from cubnm_amd import _core
+    _core.force_integrator(0 if request.param == "general" else 1)
+    yield _core
     _core.force_integrator(-1)
-    return _core
+
+
+def _batched(core):
+    return core.core.last_timings()["integrator"] == 1
 
 
 def _compare(out, ref, model, tol=TOL, check_states=True):
     rel = cases.bold_rel_err(out["sim_bold"], ref["sim_bold"])
     assert rel.max() <= tol, f"BOLD rel err {rel.max():.3e}"
     if "sim_fc_trils" in ref:
-        e = np.abs(out["sim_fc_trils"] - ref["sim_fc_trils"]).max()
+        e = cases.max_abs_err(out["sim_fc_trils"], ref["sim_fc_trils"])
         assert e <= tol, f"FC abs err {e:.3e}"
     if "sim_fcd_trils" in ref:
-        e = np.abs(out["sim_fcd_trils"] - ref["sim_fcd_trils"]).max()
+        e = cases.max_abs_err(out["sim_fcd_trils"], ref["sim_fcd_trils"])
         assert e <= tol, f"FCD abs err {e:.3e}"
     if check_states:
         s, r = out["_sim_states"], ref["_sim_states"]
-        scale = np.maximum(np.abs(r).max(axis=(1, 2), keepdims=True), 1e-300)
-        e = (np.abs(s - r) / scale).max()
+        scale = np.maximum(np.nanmax(np.abs(r), axis=(1, 2), keepdims=True), 1e-300)
+        e = cases.max_abs_err(s / scale, r / scale)
         assert e <= tol, f"states rel err {e:.3e}"
     np.testing.assert_array_equal(out["_global_bools"], ref["_global_bools"])
     np.testing.assert_array_equal(out["_global_ints"], ref["_global_ints"])
@@ -77,7 +82,9 @@ def test_kuramoto_short_horizon(core):
 
 
 @pytest.mark.parametrize("fic", [0, 1, 2])
-def test_rww_fic_modes(core, fic):
+def test_rww_fic_modes(core, fic, request):
+    if fic == 2 and request.node.callspec.params["core"] == "batched":
+        pytest.skip("numerical-FIC restarts run on the general integrator")
     N, S = 16, 4
     sc = cases.synthetic_sc(N, seed=3) * 0.05
     gp, rp = cases.params("rWW", S, N, G=[0.5, 1.0, 1.5, 2.0], do_fic=fic > 0)
@@ -90,7 +97,9 @@ def test_rww_fic_modes(core, fic):
 
 
 @pytest.mark.parametrize("model", ["rWW", "JR"])
-def test_conduction_delay(core, model):
+def test_conduction_delay(core, model, request):
+    if request.node.callspec.params["core"] == "batched":
+        pytest.skip("conduction delay runs on the general integrator")
     N, S = 12, 2
     sc = cases.synthetic_sc(N, seed=4)
     dist = cases.synthetic_sc(N, seed=5) * 60.0
@@ -101,7 +110,9 @@ def test_conduction_delay(core, model):
     _compare(out, ref, model)
 
 
-def test_states_time_series(core):
+def test_states_time_series(core, request):
+    if request.node.callspec.params["core"] == "batched":
+        pytest.skip("state time series run on the general integrator")
     N, S = 8, 2
     sc = cases.synthetic_sc(N, seed=6)
     gp, rp = cases.params("WC", S, N, G=[0.5, 1.0])
@@ -123,3 +134,29 @@ def test_c1_plain_grid_rww(core):
     args = cases.sim_args("rWW", sc, gp, rp, cfg, 8, duration_s=60)
     out, ref, _, _ = _run_both(core, args)
     _compare(out, ref, "rWW")
+
+
+@pytest.mark.parametrize("nodes", [1, 5, 16, 33, 68, 130])
+def test_batched_node_counts(core, nodes):
+    """Tile padding: node counts that are not multiples of 4 or 16, one tile,
+    several tiles, and the L2-streamed SC fragments (N > 112)."""
+    S = 19  # one full group of 16 plus a padded group
+    sc = cases.synthetic_sc(nodes, seed=nodes)
+    gp, rp = cases.params("rWW", S, nodes, G=np.linspace(0.2, 2.0, S), do_fic=False, wIE=1.0)
+    cfg = cases.model_config("rWW", do_fic=False, TR=1.0, window_size=4, window_step=2, bold_remove_s=2,
+                             do_fc=nodes > 1, do_fcd=nodes > 1)
+    args = cases.sim_args("rWW", sc, gp, rp, cfg, S, duration_s=10)
+    out, ref, _, _ = _run_both(core, args)
+    _compare(out, ref, "rWW")
+
+
+def test_batched_mixed_scs(core):
+    """Simulations with different SCs are grouped per SC."""
+    N, S = 20, 21
+    scs = np.stack([cases.synthetic_sc(N, seed=s) for s in range(3)])
+    sci = np.array([i % 3 for i in range(S)], np.intc)
+    gp, rp = cases.params("WC", S, N, G=np.linspace(0.1, 1.0, S))
+    cfg = cases.model_config("WC", TR=1.0, window_size=4, window_step=2, bold_remove_s=2)
+    args = cases.sim_args("WC", scs, gp, rp, cfg, S, duration_s=8, sc_indices=sci)
+    out, ref, _, _ = _run_both(core, args)
+    _compare(out, ref, "WC")
