@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the simulation core on one MI355X.
+
+Workload (the reference's headline, README.rst:40-48): rWW, N=100 nodes,
+32,768 simulations on a 32x32x32 grid of G in [0.001, 10], w_p in [0, 2],
+J_N in [0.001, 0.5] (itertools.product order, as GridOptimizer builds it,
+optimize.py:1220-1223), 60 s at dt 0.1 ms / BW dt 1 ms, TR 1 s, FCD window 10
+TRs step 2, 30 s burn-in, analytical FIC off.  SC: seeded lognormal(0,1),
+symmetrised, zero diagonal, max 1.  Empirical target: seeded AR(1) BOLD
+100x60 (phi 0.7, Cholesky-correlated innovations from a Wishart covariance).
+
+One step = integrate all simulations (BOLD, states), FC + FCD, and the
+FC-Pearson and FCD-KS scores against the empirical target, with inputs and
+outputs resident in HBM (cubnm_run_device).  Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "sim-node-seconds/wall-s (rWW N=100, 60 s); grad-fit vs CMA-ES time-to-loss"
+A100_PUBLISHED = 585000.0        # sim-node-s/s, 32,768 x 100 x 60 s in 5.6 min
+FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 (vector and matrix)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def grid_params(n_sims, nodes):
+    g = np.linspace(0.001, 10.0, 32)
+    wp = np.linspace(0.0, 2.0, 32)
+    jn = np.linspace(0.001, 0.5, 32)
+    GG, WW, JJ = np.meshgrid(g, wp, jn, indexing="ij")
+    G, W, J = GG.ravel(), WW.ravel(), JJ.ravel()
+    idx = np.arange(n_sims) % G.size
+    return G[idx], W[idx], J[idx]
+
+
+def empirical_bold(nodes, vols, seed=0, phi=0.7):
+    rng = np.random.default_rng(seed)
+    df = nodes + 10
+    A = rng.standard_normal((nodes, df))
+    L = np.linalg.cholesky(A @ A.T / df)
+    x = np.empty((nodes, vols))
+    x[:, 0] = L @ rng.standard_normal(nodes)
+    for t in range(1, vols):
+        x[:, t] = phi * x[:, t - 1] + np.sqrt(1 - phi ** 2) * (L @ rng.standard_normal(nodes))
+    return x
+
+
+def ops_per_sim_node_second(nodes, dt_ms, bw_dt_ms, TR_s):
+    """Algorithmic FP64 ops (exp, pow, div count as one): 2N+42 per node per
+    model step (N coupling MACs + rWW step), 24 per BW step, 10 per BOLD."""
+    return (1000.0 / dt_ms) * (2 * nodes + 42) + (1000.0 / bw_dt_ms) * 24 + 10.0 / TR_s
+
+
+def cpu_baseline(n_cores, nodes, duration_s, G, W, J, sc, cfg):
+    """The oracle (FP64 C++ restatement of the reference CPU engine, same
+    flags) on a bounded sample of the grid: one full-length simulation per
+    thread, run phase only."""
+    import cases
+    from oracle import oracle as O
+    O.lib()
+    pick = np.linspace(0, G.size - 1, n_cores).round().astype(int)
+    gp, rp = cases.params("rWW", n_cores, nodes, G=G[pick], w_p=W[pick][:, None], J_N=J[pick][:, None],
+                          do_fic=False, wIE=1.0)
+    args = cases.sim_args("rWW", sc, gp, rp, cfg, n_cores, duration_s=duration_s)
+    out = cases.run_oracle(args, n_threads=n_cores)
+    rt = out["run_time"]
+    return {
+        "value": n_cores * nodes * duration_s / rt,
+        "unit": "sim-node-s/s",
+        "cores": n_cores,
+        "kind": "port",
+        "sample": f"{n_cores} of the grid's simulations (evenly spaced), full {duration_s:g} s each, "
+                  f"one per OpenMP thread; run phase only ({rt:.2f} s), noise generation excluded",
+    }
+
+
+def read_traffic(path):
+    """Bytes per integrator launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, calibrated against a copy kernel), if committed."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch="):
+            return float(line.split("=", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sims", type=int, default=32768)
+    ap.add_argument("--nodes", type=int, default=100)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--cpu-cores", type=int, default=0, help="threads for the CPU baseline (0: all, max 16)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tpw", type=int, default=0, help="batched integrator tiles per wave (0: auto)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has one MI355X and the simulations shard "
+              "with no exchange (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cases
+    from cubnm_amd import capi, utils
+
+    dev = torch.device("cuda:0")
+    S, N, dur = args.sims, args.nodes, args.duration
+    TR_s, dt, bw_dt = 1.0, 0.1, 1.0
+    sc = cases.synthetic_sc(N, seed=0)
+    G, W, J = grid_params(S, N)
+    cfg = cases.model_config("rWW", TR=TR_s, window_size=10, window_step=2, bold_remove_s=30, do_fic=False)
+    emp = empirical_bold(N, int(dur / TR_s), seed=0)
+    emp_fc = utils.calculate_fc(emp)
+    emp_fcd = np.sort(utils.calculate_fcd(emp, 10, 2))
+
+    capi.set_tuning("batched_tpw", args.tpw)
+    capi.force_integrator(-1)
+    d = capi.prepare("rWW", cfg, S, N, int(dur * 1000), int(TR_s * 1000), int(TR_s * 1000), 0, dt, bw_dt)
+    gp, rp = cases.params("rWW", S, N, G=G, w_p=W[:, None], J_N=J[:, None], do_fic=False, wIE=1.0)
+    t = lambda a, dtype=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+    d_sc, d_sci = t(sc.reshape(1, -1)), t(np.zeros(S), torch.int32)
+    d_dist, d_gp, d_rp, d_v = t(np.zeros(N * N)), t(gp), t(rp), t(np.zeros(S))
+    d_bold = torch.empty((S, d.bold_len * N), dtype=torch.float64, device=dev)
+    d_fc = torch.empty((S, d.n_pairs), dtype=torch.float64, device=dev)
+    d_fcd = torch.empty((S, d.n_window_pairs), dtype=torch.float64, device=dev)
+    d_st = torch.empty((d.n_state_vars, S, N), dtype=torch.float64, device=dev)
+    d_gb = torch.zeros((d.n_global_out_bool, S), dtype=torch.uint8, device=dev)
+    d_gi = torch.zeros((d.n_global_out_int, S), dtype=torch.int32, device=dev)
+    d_emp_fc, d_emp_fcd = t(emp_fc), t(emp_fcd)
+    d_corr = torch.empty(S, dtype=torch.float64, device=dev)
+    d_ks = torch.empty(S, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        capi.run_device(d_sc.data_ptr(), d_sci.data_ptr(), d_dist.data_ptr(), d_gp.data_ptr(), d_rp.data_ptr(),
+                        d_v.data_ptr(), d_bold.data_ptr(), d_fc.data_ptr(), d_fcd.data_ptr(), d_st.data_ptr(),
+                        d_gb.data_ptr(), d_gi.data_ptr())
+        capi.gof_fc_corr_device(d_fc.data_ptr(), S, d.n_pairs, d_emp_fc.data_ptr(), d_corr.data_ptr())
+        capi.gof_fcd_ks_device(d_fcd.data_ptr(), S, d.n_window_pairs, d_emp_fcd.data_ptr(), emp_fcd.size,
+                               d_ks.data_ptr())
+        return capi.last_timings()
+
+    for i in range(args.warmup):
+        tm = step()
+        log(f"warmup {i}: {tm}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    integ_ms, fc_ms = [], []
+    for i in range(args.steps):
+        tm = step()
+        integ_ms.append(tm["integrate_ms"])
+        fc_ms.append(tm["fc_fcd_ms"])
+        log(f"step {i}: {tm}")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    value = S * N * dur / (ms_per_step / 1000.0)
+
+    corr, ks = d_corr.cpu().numpy(), d_ks.cpu().numpy()
+    log(f"scores: fc_corr mean {np.nanmean(corr):.4f}, fcd_ks mean {np.nanmean(ks):.4f}, "
+        f"nan {int(np.isnan(corr).sum())}/{int(np.isnan(ks).sum())}")
+    ops = ops_per_sim_node_second(N, dt, bw_dt, TR_s) * S * N * dur
+    k_ms = float(np.mean(integ_ms))
+    achieved = ops / (k_ms / 1000.0) / 1e12
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "integrate_batched_rww_n100_c2_traffic.txt"))
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
+                "kernel": "integrate_batched_kernel", "kernel_ms": round(k_ms, 2),
+                "fc_fcd_ms": round(float(np.mean(fc_ms)), 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        n_cores = args.cpu_cores or min(16, len(os.sched_getaffinity(0)))
+        cpu = cpu_baseline(n_cores, N, dur, G, W, J, sc, cfg)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "sim-node-s/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(value / A100_PUBLISHED, 3),
+        "dtype": "f64", "data": "synthetic (seeded lognormal SC, seeded AR(1) empirical BOLD, host-generated noise)",
+        "config": {"workload": f"rWW grid {S} sims x {N} nodes x {dur:g} s (dt 0.1 ms, BW dt 1 ms, TR 1 s, "
+                               f"FCD window 10 step 2, FIC off), FC + FCD + FC-Pearson + FCD-KS on GPU",
+                   "sims": S, "nodes": N, "duration_s": dur, "integrator": "batched MFMA FP64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,105 @@
+"""ctypes binding of the C ABI declared in include/cubnm_amd.h.
+
+The CPython module ``cubnm_amd.core`` is the drop-in for the reference's
+``cubnm.core``; this binding exposes the same library to callers that keep
+their arrays in device memory (``run_device``), e.g. torch CUDA tensors.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "bold_len", "n_pairs", "n_windows", "n_window_pairs", "states_len", "n_state_vars",
+        "n_global_params", "n_regional_params", "n_global_out_bool", "n_global_out_int",
+        "noise_bw_it", "noise_repeats", "bw_it", "inner_it", "n_vols_remove", "corr_len",
+        "do_fc", "do_fcd", "modifies_params", "N_SIMS", "nodes")] + [
+        ("noise_size", ctypes.c_longlong), ("init_time", ctypes.c_double)]
+
+
+class Outputs(ctypes.Structure):
+    _fields_ = [("sim_bold", ctypes.c_void_p), ("sim_fc_trils", ctypes.c_void_p),
+                ("sim_fcd_trils", ctypes.c_void_p), ("sim_states", ctypes.c_void_p),
+                ("global_bools", ctypes.c_void_p), ("global_ints", ctypes.c_void_p),
+                ("noise", ctypes.c_void_p), ("shuffled_nodes", ctypes.c_void_p),
+                ("shuffled_ts", ctypes.c_void_p), ("run_time", ctypes.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "libcubnm_amd.so")
+    if not os.path.exists(path):
+        raise ImportError("libcubnm_amd.so is not built; run `python -m cubnm_amd.build`")
+    L = ctypes.CDLL(path)
+    I, D, P, C = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_char_p
+    CP = ctypes.POINTER(ctypes.c_char_p)
+    L.cubnm_last_error.restype = C
+    L.cubnm_init.restype = I
+    L.cubnm_set_const.argtypes = [C, D]
+    L.cubnm_prepare.argtypes = [C, I, CP, CP] + [I] * 13 + [D, D, ctypes.POINTER(Dims)]
+    L.cubnm_run.argtypes = [P] * 6 + [ctypes.POINTER(Outputs)]
+    L.cubnm_run_device.argtypes = [P] * 6 + [ctypes.POINTER(Outputs)]
+    L.cubnm_last_timings.argtypes = [ctypes.POINTER(ctypes.c_double * 6)]
+    L.cubnm_force_integrator.argtypes = [I]
+    L.cubnm_last_integrator.restype = I
+    L.cubnm_set_tuning.argtypes = [C, I]
+    L.cubnm_gof_fc_corr.argtypes = [P, I, I, P, P]
+    L.cubnm_gof_fcd_ks.argtypes = [P, I, I, P, I, P]
+    L.cubnm_gof_fc_corr_device.argtypes = [P, I, I, P, P]
+    L.cubnm_gof_fcd_ks_device.argtypes = [P, I, I, P, I, P]
+    L.cubnm_init()
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().cubnm_last_error().decode())
+
+
+def prepare(model, config, N_SIMS, nodes, time_steps, BOLD_TR, states_sampling, sim_seed, dt, bw_dt,
+            N_SCs=1, ext_out=True, states_ts=False, noise_out=False, do_delay=False, force_reinit=False):
+    keys = (ctypes.c_char_p * len(config))(*[k.encode() for k in config])
+    vals = (ctypes.c_char_p * len(config))(*[str(v).encode() for v in config.values()])
+    d = Dims()
+    _check(lib().cubnm_prepare(model.encode(), len(config), keys, vals, int(ext_out), int(states_ts),
+                               int(noise_out), int(do_delay), int(force_reinit), 0, N_SIMS, nodes, N_SCs,
+                               time_steps, BOLD_TR, states_sampling, sim_seed, float(dt), float(bw_dt),
+                               ctypes.byref(d)))
+    return d
+
+
+def run_device(SC, SC_indices, SC_dist, global_params, regional_params, v_list, bold, fc=None, fcd=None,
+               states=None, global_bools=None, global_ints=None):
+    """All arguments are device addresses (ints), e.g. tensor.data_ptr()."""
+    o = Outputs(bold, fc, fcd, states, global_bools, global_ints, None, None, None, 0.0)
+    _check(lib().cubnm_run_device(SC, SC_indices, SC_dist, global_params, regional_params, v_list, ctypes.byref(o)))
+    return o.run_time
+
+
+def last_timings():
+    t = (ctypes.c_double * 6)()
+    lib().cubnm_last_timings(ctypes.byref(t))
+    keys = ["prep_ms", "h2d_ms", "integrate_ms", "fc_fcd_ms", "d2h_ms", "integrate_launches"]
+    return dict(zip(keys, list(t)))
+
+
+def set_tuning(key, value):
+    _check(lib().cubnm_set_tuning(key.encode(), int(value)))
+
+
+def force_integrator(k):
+    lib().cubnm_force_integrator(int(k))
+
+
+def gof_fc_corr_device(sim_fc, n_sims, n_pairs, emp_fc, out):
+    _check(lib().cubnm_gof_fc_corr_device(sim_fc, n_sims, n_pairs, emp_fc, out))
+
+
+def gof_fcd_ks_device(sim_fcd, n_sims, n, emp_sorted, m, out):
+    _check(lib().cubnm_gof_fcd_ks_device(sim_fcd, n_sims, n, emp_sorted, m, out))

@@ -55,8 +55,14 @@ __device__ int upper_bound(const double* a, int n, double x) {
 }  // namespace
 
 __global__ void __launch_bounds__(GOF_THREADS) fc_corr_kernel(const double* sim, int n_sims, int n, const double* emp,
-                                                              double emp_mean, double emp_ss, double* out) {
+                                                              double* out) {
     __shared__ double red[GOF_THREADS / 64];
+    double e_acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) e_acc += emp[i];
+    const double emp_mean = wg_sum(e_acc, red) / n;
+    double e_ss = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) { const double d = emp[i] - emp_mean; e_ss += d * d; }
+    const double emp_ss = wg_sum(e_ss, red);
     for (int s = blockIdx.x; s < n_sims; s += gridDim.x) {
         const double* x = sim + (long long)s * n;
         double acc = 0.0;
@@ -134,62 +140,76 @@ int guarded(F f) {
 }
 }  // namespace
 
+namespace {
+template <class F>
+int with_device_copies(const double* sim, size_t sim_n, const double* emp, size_t emp_n, double* out, size_t out_n, F f) {
+    double *d_sim = nullptr, *d_emp = nullptr, *d_out = nullptr;
+    int rc = -1;
+    if (hipMalloc(&d_sim, sizeof(double) * sim_n) == hipSuccess && hipMalloc(&d_emp, sizeof(double) * emp_n) == hipSuccess &&
+        hipMalloc(&d_out, sizeof(double) * out_n) == hipSuccess &&
+        hipMemcpy(d_sim, sim, sizeof(double) * sim_n, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(d_emp, emp, sizeof(double) * emp_n, hipMemcpyHostToDevice) == hipSuccess) {
+        rc = f(d_sim, d_emp, d_out);
+        if (rc == 0 && hipMemcpy(out, d_out, sizeof(double) * out_n, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+    } else {
+        g_gof_err = "device allocation or copy failed";
+        cubnm::set_last_error(g_gof_err);
+    }
+    if (d_sim) hipFree(d_sim);
+    if (d_emp) hipFree(d_emp);
+    if (d_out) hipFree(d_out);
+    return rc;
+}
+}  // namespace
+
 extern "C" {
+
+int cubnm_gof_fc_corr_device(const double* d_sim, int n_sims, int n_pairs, const double* d_emp, double* d_out);
+int cubnm_gof_fcd_ks_device(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m, double* d_out);
 
 const char* cubnm_gof_last_error(void) { return g_gof_err.c_str(); }
 
-int cubnm_gof_fc_corr(const double* sim_fc, int n_sims, int n_pairs, const double* emp_fc, double* out) {
-    if (n_sims <= 0 || n_pairs <= 1) return -1;
+int cubnm_gof_fc_corr_device(const double* d_sim, int n_sims, int n_pairs, const double* d_emp, double* d_out) {
+    if (n_sims <= 0 || n_pairs <= 1) { g_gof_err = "need n_sims > 0 and n_pairs > 1"; return -1; }
     return guarded([&]() -> hipError_t {
-        double em = 0.0, ss = 0.0;
-        for (int i = 0; i < n_pairs; i++) em += emp_fc[i];
-        em /= n_pairs;
-        for (int i = 0; i < n_pairs; i++) ss += (emp_fc[i] - em) * (emp_fc[i] - em);
-        double *d_sim = nullptr, *d_emp = nullptr, *d_out = nullptr;
-        hipError_t e;
-        const size_t sb = sizeof(double) * (size_t)n_sims * n_pairs;
-        if ((e = hipMalloc(&d_sim, sb)) != hipSuccess) return e;
-        if ((e = hipMalloc(&d_emp, sizeof(double) * n_pairs)) != hipSuccess) { hipFree(d_sim); return e; }
-        if ((e = hipMalloc(&d_out, sizeof(double) * n_sims)) != hipSuccess) { hipFree(d_sim); hipFree(d_emp); return e; }
-        e = hipMemcpy(d_sim, sim_fc, sb, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(d_emp, emp_fc, sizeof(double) * n_pairs, hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(cubnm::fc_corr_kernel, dim3(std::min(n_sims, 4096)), dim3(cubnm::GOF_THREADS), 0, 0,
-                               d_sim, n_sims, n_pairs, d_emp, em, ss, d_out);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipMemcpy(out, d_out, sizeof(double) * n_sims, hipMemcpyDeviceToHost);
-        hipFree(d_sim); hipFree(d_emp); hipFree(d_out);
-        return e;
+        hipLaunchKernelGGL(cubnm::fc_corr_kernel, dim3(std::min(n_sims, 4096)), dim3(cubnm::GOF_THREADS), 0, 0,
+                           d_sim, n_sims, n_pairs, d_emp, d_out);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? hipDeviceSynchronize() : e;
     });
 }
 
-int cubnm_gof_fcd_ks(const double* sim_fcd, int n_sims, int n, const double* emp_fcd, int m, double* out) {
-    if (n_sims <= 0 || n <= 0 || m <= 0) return -1;
+int cubnm_gof_fcd_ks_device(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m, double* d_out) {
+    if (n_sims <= 0 || n <= 0 || m <= 0) { g_gof_err = "need n_sims, n and m > 0"; return -1; }
     int n_pow2 = 1;
     while (n_pow2 < n) n_pow2 <<= 1;
     if ((size_t)n_pow2 * sizeof(double) > 128 * 1024) { g_gof_err = "too many FCD values per simulation for the LDS sort"; return -1; }
+    return guarded([&]() -> hipError_t {
+        hipLaunchKernelGGL(cubnm::fcd_ks_kernel, dim3(std::min(n_sims, 4096)), dim3(cubnm::GOF_THREADS),
+                           sizeof(double) * n_pow2, 0, d_sim, n_sims, n, n_pow2, d_emp_sorted, m, d_out);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? hipDeviceSynchronize() : e;
+    });
+}
+
+
+int cubnm_gof_fc_corr(const double* sim_fc, int n_sims, int n_pairs, const double* emp_fc, double* out) {
+    if (n_sims <= 0 || n_pairs <= 1) { g_gof_err = "need n_sims > 0 and n_pairs > 1"; return -1; }
+    return with_device_copies(sim_fc, (size_t)n_sims * n_pairs, emp_fc, n_pairs, out, n_sims,
+                              [&](double* ds, double* de, double* dout) {
+                                  return cubnm_gof_fc_corr_device(ds, n_sims, n_pairs, de, dout);
+                              });
+}
+
+int cubnm_gof_fcd_ks(const double* sim_fcd, int n_sims, int n, const double* emp_fcd, int m, double* out) {
+    if (n_sims <= 0 || n <= 0 || m <= 0) { g_gof_err = "need n_sims, n and m > 0"; return -1; }
     std::vector<double> emp(emp_fcd, emp_fcd + m);
     for (double x : emp) if (std::isnan(x)) { for (int i = 0; i < n_sims; i++) out[i] = NAN; return 0; }
     std::sort(emp.begin(), emp.end());
-    return guarded([&]() -> hipError_t {
-        double *d_sim = nullptr, *d_emp = nullptr, *d_out = nullptr;
-        hipError_t e;
-        const size_t sb = sizeof(double) * (size_t)n_sims * n;
-        if ((e = hipMalloc(&d_sim, sb)) != hipSuccess) return e;
-        if ((e = hipMalloc(&d_emp, sizeof(double) * m)) != hipSuccess) { hipFree(d_sim); return e; }
-        if ((e = hipMalloc(&d_out, sizeof(double) * n_sims)) != hipSuccess) { hipFree(d_sim); hipFree(d_emp); return e; }
-        e = hipMemcpy(d_sim, sim_fcd, sb, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(d_emp, emp.data(), sizeof(double) * m, hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(cubnm::fcd_ks_kernel, dim3(std::min(n_sims, 4096)), dim3(cubnm::GOF_THREADS),
-                               sizeof(double) * n_pow2, 0, d_sim, n_sims, n, n_pow2, d_emp, m, d_out);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipMemcpy(out, d_out, sizeof(double) * n_sims, hipMemcpyDeviceToHost);
-        hipFree(d_sim); hipFree(d_emp); hipFree(d_out);
-        return e;
-    });
+    return with_device_copies(sim_fcd, (size_t)n_sims * n, emp.data(), m, out, n_sims,
+                              [&](double* ds, double* de, double* dout) {
+                                  return cubnm_gof_fcd_ks_device(ds, n_sims, n, de, m, dout);
+                              });
 }
 
 }  // extern "C"

@@ -116,6 +116,14 @@ int cubnm_gof_fc_corr(const double* sim_fc, int n_sims, int n_pairs, const doubl
 int cubnm_gof_fcd_ks(const double* sim_fcd, int n_sims, int n_sim_values, const double* emp_fcd,
                      int n_emp_values, double* out);
 
+/* Device-pointer forms of the two GOF reductions (inputs and outputs resident
+ * in HBM); the empirical FCD values must already be sorted ascending.  Both
+ * return after the kernel has completed. */
+int cubnm_gof_fc_corr_device(const double* d_sim_fc, int n_sims, int n_pairs, const double* d_emp_fc,
+                             double* d_out);
+int cubnm_gof_fcd_ks_device(const double* d_sim_fcd, int n_sims, int n_sim_values,
+                            const double* d_emp_fcd_sorted, int n_emp_values, double* d_out);
+
 #ifdef __cplusplus
 }
 #endif
