@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0, help="threads for the CPU baseline (0: all, max 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tpw", type=int, default=0, help="batched integrator tiles per wave (0: auto)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X and the simulations shard "
@@ -154,6 +156,15 @@ def main():
                                d_ks.data_ptr())
         return capi.last_timings()
 
+    if args.copy_calibration_mib:
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads and writes n*8 bytes
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_calibration_mib} MiB read + {args.copy_calibration_mib} MiB written")
+        del src, dst
     for i in range(args.warmup):
         tm = step()
         log(f"warmup {i}: {tm}")

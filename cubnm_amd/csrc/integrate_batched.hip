@@ -35,8 +35,10 @@ struct BatchedArgs {
     IntegrateArgs base;
     const int* group_sims;  // [n_groups][16], -1 = padding lane
     const double* afrag;    // [N_SCs][NT][KT][64] SC^T in MFMA A-fragment order
-    int n_groups, KT, NT;
+    int n_groups, KT, NT;   // KT: fragments per tile (even), NT: tiles (multiple of TPW)
 };
+
+int batched_kt(int nodes);
 
 namespace {
 
@@ -56,14 +58,15 @@ __device__ __forceinline__ void publish(double* xs, int kn, int j, int c, double
 }  // namespace
 
 // A_LDS: A fragments staged in LDS (small N) or read from global memory.
-template <class M, int TPW, int MAXW, bool A_LDS>
+// KTC: compile-time fragment count per tile (0 = runtime ba.KT).
+template <class M, int TPW, int MAXW, bool A_LDS, int KTC>
 __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArgs ba, typename M::C mc) {
     constexpr int NS = 4 * TPW;                 // node slots per lane
     constexpr int NX = M::is_osc ? 2 : 1;        // B operands per step
     const IntegrateArgs& a = ba.base;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, q = lane >> 4;
-    const int N = a.nodes, KT = ba.KT, KN = 4 * KT, NT = ba.NT;
+    const int N = a.nodes, KT = KTC > 0 ? KTC : ba.KT, KN = 4 * KT, NT = ba.NT;
     const int buf_stride = NX * KN * 16;        // doubles per X buffer
     extern __shared__ double lds[];
     double* xs = lds;                            // [2][NX][KN][16]
@@ -121,7 +124,17 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
             // states are summed from the last inner step of a sampling bw step
             const bool sample = ((bw_i + 1) % a.states_sampling_iters == 0) &&
                                 (states_i >= a.n_states_samples_remove) && sim_ok;
+            const double* nz = a.noise + (long long)sh_t * a.inner_it * N * M::n_noise;
             for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
+                // this step's noise: issued before the coupling product so the
+                // loads complete under the MFMAs
+                double n0[NS], n1[NS];
+#pragma unroll
+                for (int sl = 0; sl < NS; sl++) {
+                    const double* p = nz + ((long long)inner_i * N + shj[sl]) * M::n_noise;
+                    n0[sl] = p[0];
+                    n1[sl] = (M::n_noise > 1) ? p[1] : 0.0;
+                }
                 const double* xb = xs + buf * buf_stride;
                 v4d acc[TPW][NX];
 #pragma unroll
@@ -129,21 +142,50 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
 #pragma unroll
                     for (int x = 0; x < NX; x++) acc[t][x] = v4d{0.0, 0.0, 0.0, 0.0};
                 const double* ab = abase + (long long)(wave * TPW) * KT * 64 + lane;
-#pragma unroll 5
-                for (int kt = 0; kt < KT; kt++) {
-                    double af[TPW];
+                const double* bb = xb + lane;
+                if constexpr (KTC > 0) {
+                    // compile-time fragment count: fully unrolled, immediate LDS offsets
 #pragma unroll
-                    for (int t = 0; t < TPW; t++) af[t] = ab[(t * KT + kt) * 64];
+                    for (int kt = 0; kt < KTC; kt++) {
 #pragma unroll
-                    for (int x = 0; x < NX; x++) {
-                        const double b = xb[x * KN * 16 + kt * 64 + lane];
+                        for (int x = 0; x < NX; x++) {
+                            const double b = bb[x * KN * 16 + kt * 64];
 #pragma unroll
-                        for (int t = 0; t < TPW; t++)
-                            acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], b, acc[t][x], 0, 0, 0);
+                            for (int t = 0; t < TPW; t++)
+                                acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[(t * KT + kt) * 64], b, acc[t][x], 0, 0, 0);
+                        }
+                    }
+                } else {
+                    // runtime fragment count: software-pipelined two k-steps ahead
+                    double a0[TPW], a1[TPW], b0[NX], b1[NX];
+#pragma unroll
+                    for (int t = 0; t < TPW; t++) { a0[t] = ab[t * KT * 64]; a1[t] = ab[(t * KT + 1) * 64]; }
+#pragma unroll
+                    for (int x = 0; x < NX; x++) { b0[x] = bb[x * KN * 16]; b1[x] = bb[x * KN * 16 + 64]; }
+                    for (int kt = 0; kt < KT; kt += 2) {
+                        const int k2 = min(kt + 2, KT - 2), k3 = k2 + 1;
+                        double a2[TPW], a3[TPW], b2[NX], b3[NX];
+#pragma unroll
+                        for (int t = 0; t < TPW; t++) { a2[t] = ab[(t * KT + k2) * 64]; a3[t] = ab[(t * KT + k3) * 64]; }
+#pragma unroll
+                        for (int x = 0; x < NX; x++) { b2[x] = bb[x * KN * 16 + k2 * 64]; b3[x] = bb[x * KN * 16 + k3 * 64]; }
+                        // keep the prefetch ahead of the MFMAs consuming the previous pair
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int x = 0; x < NX; x++)
+#pragma unroll
+                            for (int t = 0; t < TPW; t++) {
+                                acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0[x], acc[t][x], 0, 0, 0);
+                                acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1[x], acc[t][x], 0, 0, 0);
+                            }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int t = 0; t < TPW; t++) { a0[t] = a2[t]; a1[t] = a3[t]; }
+#pragma unroll
+                        for (int x = 0; x < NX; x++) { b0[x] = b2[x]; b1[x] = b3[x]; }
                     }
                 }
                 double* xn = xs + (buf ^ 1) * buf_stride;
-                const long long nbase = (long long)(sh_t * a.inner_it + inner_i) * N;
                 const bool last = sample && (inner_i == a.inner_it - 1);
 #pragma unroll
                 for (int t = 0; t < TPW; t++)
@@ -154,10 +196,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                         double gin;
                         if constexpr (M::is_osc) gin = cs[sl] * acc[t][0][r] - sn[sl] * acc[t][1][r];
                         else gin = acc[t][0][r];
-                        const long long nidx = (nbase + shj[sl]) * M::n_noise;
-                        const double n0 = a.noise[nidx];
-                        const double n1 = (M::n_noise > 1) ? a.noise[nidx + 1] : 0.0;
-                        M::step(mc, s[sl], gp, rp[sl], gin, n0, n1);
+                        M::step(mc, s[sl], gp, rp[sl], gin, n0[sl], n1[sl]);
                         if (j < N) {
                             publish<M>(xn, KN, j, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
                             if (last)
@@ -170,7 +209,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                 __syncthreads();
             }
 #pragma unroll
-            for (int sl = 0; sl < NS; sl++) bw_step(a.bw, s[sl][M::bold_idx], bx[sl], bf[sl], bn[sl], bq[sl]);
+            for (int sl = 0; sl < NS; sl++) bw_step_explog(a.bw, s[sl][M::bold_idx], bx[sl], bf[sl], bn[sl], bq[sl]);
             if ((bw_i + 1) % a.BOLD_TR_iters == 0) {
                 if (sim_ok && bold_i < a.bold_len)
 #pragma unroll
@@ -224,14 +263,15 @@ __global__ void build_afrag_kernel(const double* SC, int n_scs, int N, int NT, i
     }
 }
 
-template <class M, int TPW, int MAXW, bool A_LDS>
+template <class M, int TPW, int MAXW, bool A_LDS, int KTC = 0>
 hipError_t launch_one(const BatchedArgs& ba, const void* consts, int waves, hipStream_t st) {
     if (waves > MAXW) return hipErrorInvalidConfiguration;
     const typename M::C mc = *static_cast<const typename M::C*>(consts);
     const int nx = M::is_osc ? 2 : 1;
     size_t lds = sizeof(double) * 2 * nx * 4 * ba.KT * 16;
     if (A_LDS) lds += sizeof(double) * (size_t)ba.NT * ba.KT * 64;
-    auto kern = integrate_batched_kernel<M, TPW, MAXW, A_LDS>;
+    if (KTC > 0 && ba.KT != KTC) return hipErrorInvalidConfiguration;
+    auto kern = integrate_batched_kernel<M, TPW, MAXW, A_LDS, KTC>;
     const int threads = 64 * waves;
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds);
@@ -253,6 +293,7 @@ hipError_t dispatch(const BatchedArgs& ba, const void* consts, int tpw, hipStrea
     const bool a_lds = lds_a <= 150 * 1024;
     if (tpw == 1) {
         if (waves <= 4) return a_lds ? launch_one<M, 1, 4, true>(ba, consts, waves, st) : launch_one<M, 1, 4, false>(ba, consts, waves, st);
+        if (waves <= 8 && a_lds && ba.KT == 26) return launch_one<M, 1, 8, true, 26>(ba, consts, waves, st);  // N = 101..104 and 97..100
         if (waves <= 8) return a_lds ? launch_one<M, 1, 8, true>(ba, consts, waves, st) : launch_one<M, 1, 8, false>(ba, consts, waves, st);
         return hipErrorInvalidValue;
     }
@@ -281,13 +322,17 @@ void batched_plan(int nodes, int* tpw, int* nt_pad) {
     *nt_pad = waves * t;
 }
 
+// Source-node fragments per tile: ceil(nodes/4) rounded up to even (the
+// coupling loop consumes fragments in pairs; padded fragments are zero).
+int batched_kt(int nodes) { return (((nodes + 3) / 4) + 1) & ~1; }
+
 size_t batched_afrag_doubles(int nodes, int nt_pad, int n_scs) {
-    const int KT = (nodes + 3) / 4;
+    const int KT = batched_kt(nodes);
     return (size_t)n_scs * nt_pad * KT * 64;
 }
 
 hipError_t build_afrag(const double* d_SC, int n_scs, int nodes, int nt_pad, double* d_afrag, hipStream_t s) {
-    const int KT = (nodes + 3) / 4, NT = nt_pad;
+    const int KT = batched_kt(nodes), NT = nt_pad;
     const long long total = (long long)n_scs * NT * KT * 64;
     const int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
     hipLaunchKernelGGL(build_afrag_kernel, dim3(blocks), dim3(256), 0, s, d_SC, n_scs, nodes, NT, KT, d_afrag);
@@ -301,7 +346,7 @@ hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const voi
     ba.group_sims = d_group_sims;
     ba.afrag = d_afrag;
     ba.n_groups = n_groups;
-    ba.KT = (a.nodes + 3) / 4;
+    ba.KT = batched_kt(a.nodes);
     int tpw, nt_pad;
     batched_plan(a.nodes, &tpw, &nt_pad);
     ba.NT = nt_pad;

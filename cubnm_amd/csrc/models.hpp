@@ -23,6 +23,7 @@ enum ModelId { MODEL_RWW = 0, MODEL_JR = 1, MODEL_KURAMOTO = 2, MODEL_WC = 3, MO
 struct BWConsts {
     double dt, rho, alpha, tau, y, kappa, V_0, k1, k2, k3;
     double ialpha, itau, oneminrho, dt_itau;
+    double log_oneminrho;  // log(1 - rho), for the exp/log form of the BW step
 };
 
 // rWW ---------------------------------------------------------------------
@@ -155,6 +156,20 @@ CUBNM_HD void bw_step(const BWConsts& b, double s, double& x, double& f, double&
     const double tmp_f = f + b.dt * x;
     nu = nu + b.dt_itau * (f - pow(nu, b.ialpha));
     q = q + b.dt_itau * (f * (1.0 - pow(b.oneminrho, (1.0 / f))) / b.rho - pow(nu, b.ialpha) * q / nu);
+    f = tmp_f;
+}
+
+// The same BW step with the powers evaluated as exp(y * log(x)):
+// nu^(1/alpha) = exp(ialpha * log(nu)) and (1-rho)^(1/f) = exp(log(1-rho) / f).
+// This matches pow to a few ulp (the BW system is contracting, so the
+// difference stays at rounding level) and costs a third of the device pow.
+// Special values follow pow: log(0) = -inf -> 0, log(x<0) = NaN -> NaN.
+__device__ inline void bw_step_explog(const BWConsts& b, double s, double& x, double& f, double& nu, double& q) {
+    x = x + b.dt * (s - b.kappa * x - b.y * (f - 1.0));
+    const double tmp_f = f + b.dt * x;
+    nu = nu + b.dt_itau * (f - exp(b.ialpha * log(nu)));
+    const double nu_pow = exp(b.ialpha * log(nu));
+    q = q + b.dt_itau * (f * (1.0 - exp(b.log_oneminrho / f)) / b.rho - nu_pow * q / nu);
     f = tmp_f;
 }
 

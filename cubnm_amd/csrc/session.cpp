@@ -479,6 +479,7 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
     a.bold = io.bold; a.states = io.states; a.gbool = io.gb; a.gint = io.gi;
     a.bw = g_bw;
     a.bw.dt = s.bw_dt; a.bw.dt_itau = a.bw.dt * a.bw.itau;  // core.cpp:291-292
+    a.bw.log_oneminrho = std::log(a.bw.oneminrho);
     bool any_delay = false;
     if (s.do_delay) {
         int mmax = 0;
