@@ -2,7 +2,7 @@
  * cubnm_amd — C ABI of the MI355X simulation core.
  *
  * The reference exposes its core as the CPython extension `cubnm.core`
- * (src/ext/core.cpp:507-626) with three functions: run_simulations, init and
+ * (src/ext/core.cpp:495-626) with three functions: run_simulations, init and
  * set_const.  This header is the plain-C form of that boundary: pointers,
  * sizes and strings only.  `cubnm_amd.core` (a CPython module built on top of
  * this library) re-exports the exact Python signatures, and ctypes / cgo /
@@ -101,8 +101,8 @@ int cubnm_last_timings(double* out6);
 int cubnm_force_integrator(int force);
 int cubnm_last_integrator(void);
 
-/* Launch-shape knobs of the batched integrator, set by the autotuner
- * (cubnm_amd/tune.py): "batched_tpw" = 16-node tiles per wave (0 = auto, 1, 2). */
+/* Launch-shape knobs of the batched integrator (no counterpart in the
+ * reference): "batched_tpw" = 16-node tiles per wave (0 = auto, 1, 2). */
 int cubnm_set_tuning(const char* key, int value);
 
 const char* cubnm_last_error(void);

@@ -2,7 +2,7 @@
 
 * libcubnm_amd.so loads and exports every function include/cubnm_amd.h
   declares.
-* cubnm_amd.core exposes the reference's three functions (core.cpp:507-616)
+* cubnm_amd.core exposes the reference's three functions (core.cpp:147-192, 495-626)
   and raises (instead of returning None) on the reference's error paths that
   are decided on the host before any device work: use_cpu, unknown model,
   malformed arguments, too few BOLD volumes after burn-in, zero FCD windows,
