@@ -1,0 +1,44 @@
+"""SimGroup layer end to end on the GPU core: the same group run through the
+HIP core and through the oracle gives the same outputs (BOLD norm-wise
+relative <= 1e-9, FC/FCD absolute <= 1e-9, FIC wIE relative <= 1e-12), and
+score() on the GPU equals score(force_cpu=True) (SciPy)."""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _group(model_cls, **kw):
+    sg = model_cls(duration=20, TR=1, sc=cases.synthetic_sc(24, seed=5), window_size=4, window_step=2,
+                   bold_remove_s=4, **kw)
+    sg.N = 20
+    sg.param_lists["G"] = np.linspace(0.2, 2.0, 20)
+    sg._set_default_params()
+    return sg
+
+
+@pytest.mark.parametrize("model", ["rWW", "JR", "WC"])
+def test_simgroup_gpu_vs_oracle(model, monkeypatch):
+    import cubnm_amd.sim as sim
+    from cubnm_amd import _core
+    cls = {"rWW": sim.rWWSimGroup, "JR": sim.JRSimGroup, "WC": sim.WCSimGroup}[model]
+    gpu = _group(cls)
+    gpu.run()
+    ref = _group(cls)
+    with monkeypatch.context() as m:
+        m.setattr(_core, "run_simulations", lambda *a: cases.run_oracle(a))
+        ref.run()
+    assert cases.bold_rel_err(gpu.sim_bold, ref.sim_bold).max() <= 1e-9
+    assert cases.max_abs_err(gpu.sim_fc_trils, ref.sim_fc_trils) <= 1e-9
+    assert cases.max_abs_err(gpu.sim_fcd_trils, ref.sim_fcd_trils) <= 1e-9
+    if model == "rWW":
+        np.testing.assert_allclose(gpu.param_lists["wIE"], ref.param_lists["wIE"], rtol=1e-12)
+        np.testing.assert_array_equal(gpu.fic_unstable, ref.fic_unstable)
+    rng = np.random.default_rng(1)
+    emp_fc = np.tanh(rng.standard_normal(gpu.sim_fc_trils.shape[1]))
+    emp_fcd = rng.uniform(-0.2, 1, 45)
+    s_gpu = gpu.score(emp_fc, emp_fcd)
+    s_cpu = gpu.score(emp_fc, emp_fcd, force_cpu=True)
+    np.testing.assert_allclose(s_gpu.to_numpy(float), s_cpu.to_numpy(float), rtol=0, atol=1e-12)
