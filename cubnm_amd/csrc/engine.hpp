@@ -56,5 +56,14 @@ struct FcArgs {
 // Host-side launchers (defined in the .hip files).
 hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* model_consts, hipStream_t s);
 hipError_t launch_fc_fcd(const FcArgs& a, hipStream_t s);
+// true when the per-simulation LDS kernel can hold a simulation's BOLD
+bool fc_fcd_fits_lds(const FcArgs& a);
+// FC/FCD as Gram products on MFMA (FP64, or split-BF16 when split_bf16 != 0);
+// pair_of: [nodes*nodes] tril index of pair (i > j) or -1, or null for the
+// plain tril order; work: device scratch of work_bytes (at least one
+// simulation's fc_gram_workspace_bytes).
+size_t fc_gram_workspace_bytes(const FcArgs& a, int max_win_len, int split_bf16, int batch);
+hipError_t launch_fc_gram(const FcArgs& a, int max_win_len, int split_bf16, const int* pair_of, void* work,
+                          size_t work_bytes, hipStream_t s);
 
 }  // namespace cubnm

@@ -103,13 +103,13 @@ __global__ void __launch_bounds__(FC_THREADS) fc_fcd_kernel(FcArgs a) {
     }
 }
 
+bool fc_fcd_fits_lds(const FcArgs& a) {
+    return a.n_windows <= 1024 && sizeof(double) * (size_t)a.bold_len * a.nodes <= 150 * 1024;
+}
+
 hipError_t launch_fc_fcd(const FcArgs& a, hipStream_t s) {
-    if (a.n_windows > 1024) return hipErrorInvalidValue;
-    int max_rows = a.bold_len - a.n_vols_remove;
-    for (int w = 0; w < a.n_windows; w++) {}  // window rows never exceed bold_len
-    size_t lds = sizeof(double) * (size_t)a.bold_len * a.nodes;
-    (void)max_rows;
-    if (lds > 150 * 1024) return hipErrorInvalidValue;
+    if (!fc_fcd_fits_lds(a)) return hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * (size_t)a.bold_len * a.nodes;
     hipLaunchKernelGGL(fc_fcd_kernel, dim3(a.n_slots), dim3(FC_THREADS), lds, s, a);
     return hipGetLastError();
 }

@@ -235,9 +235,20 @@ PyObject* py_gof_fcd_ks(PyObject*, PyObject* args) {
 PyObject* py_last_timings(PyObject*, PyObject*) {
     double t[6];
     cubnm_last_timings(t);
-    return Py_BuildValue("{s:d,s:d,s:d,s:d,s:d,s:d,s:i}", "prep_ms", t[0], "h2d_ms", t[1], "integrate_ms", t[2],
+    return Py_BuildValue("{s:d,s:d,s:d,s:d,s:d,s:d,s:i,s:i}", "prep_ms", t[0], "h2d_ms", t[1], "integrate_ms", t[2],
                          "fc_fcd_ms", t[3], "d2h_ms", t[4], "integrate_launches", t[5], "integrator",
-                         cubnm_last_integrator());
+                         cubnm_last_integrator(), "fc_path", cubnm_last_fc_path());
+}
+
+PyObject* py_set_tuning(PyObject*, PyObject* args) {
+    const char* key;
+    int v;
+    if (!PyArg_ParseTuple(args, "si", &key, &v)) return nullptr;
+    if (cubnm_set_tuning(key, v) != 0) {
+        PyErr_SetString(PyExc_ValueError, cubnm_last_error());
+        return nullptr;
+    }
+    Py_RETURN_NONE;
 }
 
 PyObject* py_force_integrator(PyObject*, PyObject* args) {
@@ -259,6 +270,8 @@ PyMethodDef methods[] = {
     {"gof_fcd_ks", py_gof_fcd_ks, METH_VARARGS, "gof_fcd_ks(sim_fcd_trils, emp_fcd_tril) -> KS statistic per simulation"},
     {"last_timings", py_last_timings, METH_NOARGS, "last_timings() -> dict of phase timings of the last run (ms)"},
     {"force_integrator", py_force_integrator, METH_VARARGS, "force_integrator(k): -1 auto, 0 general, 1 batched"},
+    {"set_tuning", py_set_tuning, METH_VARARGS,
+     "set_tuning(key, value): 'batched_tpw' 0/1/2; 'fc_mode' 0 auto, 1 FP64 Gram, 2 split-BF16 Gram"},
     {nullptr, nullptr, 0, nullptr}};
 
 PyModuleDef module_def = {PyModuleDef_HEAD_INIT, "core", "MI355X simulation core (drop-in for cubnm.core)", -1, methods};

@@ -126,8 +126,10 @@ struct Session {
     // device buffers
     DevBuf d_noise, d_shn, d_sht, d_pairs, d_wins, d_wpairs;
     DevBuf d_SC, d_SCi, d_gp, d_rp, d_v, d_bold, d_fc, d_fcd, d_states, d_gb, d_gi, d_delays, d_maxd, d_hist, d_wfc;
-    DevBuf d_groups, d_afrag;
+    DevBuf d_groups, d_afrag, d_pair_of, d_gram;
     std::vector<int> groups_h;
+    int fc_mode = 0;          // 0 auto (LDS kernel when a simulation fits), 1 FP64 Gram, 2 split-BF16 Gram
+    int last_fc_path = -1;    // 0 LDS kernel, 1 FP64 Gram, 2 split-BF16 Gram
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     double timings[6] = {0, 0, 0, 0, 0, 0};
@@ -527,10 +529,34 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
         f.w_starts = s.d_wins.as<int>(); f.w_ends = s.d_wins.as<int>() + s.n_windows;
         f.wpairs_i = s.d_wpairs.as<int>(); f.wpairs_j = s.d_wpairs.as<int>() + s.n_window_pairs;
         f.fc = io.fc; f.fcd = io.fcd;
-        f.n_slots = std::min(s.N_SIMS, 2048);
-        s.d_wfc.ensure(sizeof(double) * std::max<size_t>(1, (size_t)f.n_slots * f.n_windows * f.n_pairs));
-        f.wfc_scratch = s.d_wfc.as<double>();
-        check(launch_fc_fcd(f, s.stream), "fc_fcd launch");
+        int path = s.fc_mode;
+        if (path == 0) path = fc_fcd_fits_lds(f) ? 0 : 1;
+        s.last_fc_path = path;
+        if (path == 0) {
+            f.n_slots = std::min(s.N_SIMS, 2048);
+            s.d_wfc.ensure(sizeof(double) * std::max<size_t>(1, (size_t)f.n_slots * f.n_windows * f.n_pairs));
+            f.wfc_scratch = s.d_wfc.as<double>();
+            check(launch_fc_fcd(f, s.stream), "fc_fcd launch");
+        } else {
+            int max_win = 1;
+            for (int w = 0; w < s.n_windows; w++) max_win = std::max(max_win, s.w_ends[w] - s.w_starts[w] + 1);
+            const int* pair_of = nullptr;
+            if (s.conf.exc_interhemispheric) {
+                std::vector<int> map((size_t)N * N, -1);
+                for (int p = 0; p < s.n_pairs; p++) map[(size_t)s.pairs_i[p] * N + s.pairs_j[p]] = p;
+                s.d_pair_of.ensure(sizeof(int) * map.size());
+                check(hipMemcpyAsync(s.d_pair_of.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice,
+                                     s.stream), "pair map upload");
+                check(hipStreamSynchronize(s.stream), "pair map upload");
+                pair_of = s.d_pair_of.as<int>();
+            }
+            const size_t per_sim = fc_gram_workspace_bytes(f, max_win, path == 2, 1);
+            const size_t cap = std::max(per_sim, (size_t)4 << 30);  // scratch cap: 4 GiB or one simulation
+            const size_t want = std::min(cap, per_sim * (size_t)s.N_SIMS);
+            s.d_gram.ensure(want);
+            check(launch_fc_gram(f, max_win, path == 2, pair_of, s.d_gram.p, s.d_gram.bytes, s.stream),
+                  "fc_gram launch");
+        }
     }
     check(hipEventRecord(s.ev[4], s.stream), "event");
     s.timings[5] = launches;
@@ -575,10 +601,16 @@ int cubnm_set_tuning(const char* key, int value) {
         g_batched_tpw = value;
         return 0;
     }
+    if (std::strcmp(key, "fc_mode") == 0) {
+        if (value < 0 || value > 2) { g_err = "fc_mode must be 0 (auto), 1 (FP64 Gram) or 2 (split-BF16 Gram)"; return -1; }
+        g_s.fc_mode = value;
+        return 0;
+    }
     g_err = std::string("unknown tuning key ") + key;
     return -1;
 }
 int cubnm_last_integrator(void) { return g_s.last_integrator; }
+int cubnm_last_fc_path(void) { return g_s.last_fc_path; }
 
 int cubnm_last_timings(double* out6) {
     for (int i = 0; i < 6; i++) out6[i] = g_s.timings[i];

@@ -101,9 +101,17 @@ int cubnm_last_timings(double* out6);
 int cubnm_force_integrator(int force);
 int cubnm_last_integrator(void);
 
-/* Launch-shape knobs of the batched integrator (no counterpart in the
- * reference): "batched_tpw" = 16-node tiles per wave (0 = auto, 1, 2). */
+/* Launch-shape and arithmetic knobs (no counterpart in the reference):
+ *   "batched_tpw" = 16-node tiles per wave of the batched integrator
+ *                   (0 = auto, 1, 2);
+ *   "fc_mode"     = FC/FCD path: 0 = auto (per-simulation LDS kernel when a
+ *                   simulation's BOLD fits in LDS, else FP64 Gram products),
+ *                   1 = FP64 Gram products on MFMA, 2 = split-BF16 Gram
+ *                   products (3 BF16 MFMA terms, FP32 accumulation flushed to
+ *                   FP64; normalisations in FP64). */
 int cubnm_set_tuning(const char* key, int value);
+/* FC/FCD path of the last run: 0 LDS kernel, 1 FP64 Gram, 2 split-BF16 Gram. */
+int cubnm_last_fc_path(void);
 
 const char* cubnm_last_error(void);
 
