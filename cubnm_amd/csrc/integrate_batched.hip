@@ -313,9 +313,9 @@ bool batched_supported(int model, int nodes) {
 
 // Tiles per wave and the padded tile count (waves * TPW) the fragment array
 // and the kernel use; every wave then owns exactly TPW allocated tiles.
-void batched_plan(int nodes, int* tpw, int* nt_pad) {
+void batched_plan(int nodes, int want_tpw, int* tpw, int* nt_pad) {
     const int nt = (nodes + 15) / 16;
-    int t = g_batched_tpw;
+    int t = want_tpw != 0 ? want_tpw : g_batched_tpw;
     if (t != 1 && t != 2) t = (nt > 8) ? 2 : 1;
     const int waves = (nt + t - 1) / t;
     *tpw = t;
@@ -340,7 +340,8 @@ hipError_t build_afrag(const double* d_SC, int n_scs, int nodes, int nt_pad, dou
 }
 
 hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
-                                    int* launches, const int* d_group_sims, int n_groups, const double* d_afrag) {
+                                    int* launches, const int* d_group_sims, int n_groups, const double* d_afrag,
+                                    int want_tpw) {
     BatchedArgs ba;
     ba.base = a;
     ba.group_sims = d_group_sims;
@@ -348,7 +349,7 @@ hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const voi
     ba.n_groups = n_groups;
     ba.KT = batched_kt(a.nodes);
     int tpw, nt_pad;
-    batched_plan(a.nodes, &tpw, &nt_pad);
+    batched_plan(a.nodes, want_tpw, &tpw, &nt_pad);
     ba.NT = nt_pad;
     *launches = 1;
     switch (model) {

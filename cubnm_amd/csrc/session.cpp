@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <tuple>
 #include <numeric>
 #include <random>
 #include <stdexcept>
@@ -29,9 +31,10 @@
 namespace cubnm {
 
 hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
-                                    int* launches, const int* d_group_sims, int n_groups, const double* d_afrag);
+                                    int* launches, const int* d_group_sims, int n_groups, const double* d_afrag,
+                                    int want_tpw);
 bool batched_supported(int model, int nodes);
-void batched_plan(int nodes, int* tpw, int* nt_pad);
+void batched_plan(int nodes, int want_tpw, int* tpw, int* nt_pad);
 size_t batched_afrag_doubles(int nodes, int nt_pad, int n_scs);
 hipError_t build_afrag(const double* d_SC, int n_scs, int nodes, int nt_pad, double* d_afrag, hipStream_t s);
 extern int g_batched_tpw;
@@ -100,6 +103,29 @@ struct BaseConf {
     bool ext_out = true, states_ts = false;
 };
 
+// ---------------------------------------------------------------- noise cache
+// The noise segment depends only on (seed, nodes, noise width, segment
+// length, inner steps per BW step); sessions that agree on these share one
+// host and one device copy.  The generator state right after the normals is
+// kept so each session can draw its own shuffles (whose count depends on the
+// duration) exactly as one uninterrupted stream would (bnm.cu:1207-1263).
+struct NoiseKey {
+    int seed, nodes, n_noise, noise_bw_it, inner_it;
+    bool operator<(const NoiseKey& o) const {
+        return std::tie(seed, nodes, n_noise, noise_bw_it, inner_it) <
+               std::tie(o.seed, o.nodes, o.n_noise, o.noise_bw_it, o.inner_it);
+    }
+    bool operator==(const NoiseKey& o) const { return !(*this < o) && !(o < *this); }
+};
+
+struct NoiseEntry {
+    std::vector<double> noise;
+    std::mt19937 after;
+    DevBuf d_noise;
+};
+
+std::map<NoiseKey, std::weak_ptr<NoiseEntry>> g_noise_cache;
+
 // ---------------------------------------------------------------- session
 struct Session {
     const ModelInfo* model = nullptr;
@@ -116,27 +142,35 @@ struct Session {
     int bw_it = 0, inner_it = 0, n_states_samples_remove = 0, n_vols_remove = 0, corr_len = 0;
     int n_pairs = 0, n_windows = 0, n_window_pairs = 0;
     std::vector<int> w_starts, w_ends, pairs_i, pairs_j, wpairs_i, wpairs_j;
-    // noise cache
+    // noise: the segment is shared through the process-wide cache; the
+    // per-repeat shuffles belong to the session (their count depends on the
+    // duration)
     int noise_bw_it = 0, noise_repeats = 0;
     long long noise_size = 0;
-    int last_seed = -1, last_time_steps = -1, last_nodes = -1, last_noise_ts = -1, last_n_noise = -1;
-    bool noise_ready = false;
-    std::vector<double> noise;
+    std::shared_ptr<NoiseEntry> noise_e;
+    NoiseKey noise_key{};
+    int shuffled_repeats = -1;
     std::vector<int> sh_nodes, sh_ts;
+    int tpw = 0;              // batched tiles per wave for this session (0 = global knob)
     // device buffers
-    DevBuf d_noise, d_shn, d_sht, d_pairs, d_wins, d_wpairs;
+    DevBuf d_shn, d_sht, d_pairs, d_wins, d_wpairs;
     DevBuf d_SC, d_SCi, d_gp, d_rp, d_v, d_bold, d_fc, d_fcd, d_states, d_gb, d_gi, d_delays, d_maxd, d_hist, d_wfc;
     DevBuf d_groups, d_afrag, d_pair_of, d_gram;
     std::vector<int> groups_h;
-    int fc_mode = 0;          // 0 auto (LDS kernel when a simulation fits), 1 FP64 Gram, 2 split-BF16 Gram
     int last_fc_path = -1;    // 0 LDS kernel, 1 FP64 Gram, 2 split-BF16 Gram
+    std::vector<unsigned char> gb_host;   // staged FIC flags (alive until the run completes)
+    std::vector<int> delays_h, maxd_h;    // staged delays (alive until the run completes)
+    double t_start = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     double timings[6] = {0, 0, 0, 0, 0, 0};
-    int force_integrator = -1, last_integrator = -1;
+    int last_integrator = -1;
 };
 
 Session g_s;
+std::vector<std::unique_ptr<Session>> g_pool;   // sessions of the grouped entry point
+int g_force_integrator = -1;  // -1 auto, 0 general, 1 batched
+int g_fc_mode = 0;            // 0 auto (LDS kernel when a simulation fits), 1 FP64 Gram, 2 split-BF16 Gram
 
 int parse_int(const std::string& v) { return std::stoi(v); }
 
@@ -247,14 +281,29 @@ void ensure_noise(Session& s, bool force) {
     if (s.noise_bw_it <= 0) throw std::runtime_error("noise segment shorter than one BW step");
     s.noise_size = (long long)s.nodes * s.noise_bw_it * s.inner_it * n_noise;
     s.noise_repeats = (int)std::ceil((double)(s.bw_it) / (double)(s.noise_bw_it));
-    const bool stale = !s.noise_ready || force || s.sim_seed != s.last_seed || s.time_steps != s.last_time_steps ||
-                       s.nodes != s.last_nodes || s.conf.noise_time_steps != s.last_noise_ts ||
-                       n_noise != s.last_n_noise;
-    if (!stale) return;
-    std::mt19937 gen(s.sim_seed);
-    std::normal_distribution<double> nd(0, 1);
-    s.noise.resize(s.noise_size);
-    for (long long i = 0; i < s.noise_size; i++) s.noise[i] = nd(gen);
+    const NoiseKey key{s.sim_seed, s.nodes, n_noise, s.noise_bw_it, s.inner_it};
+    if (force || !s.noise_e || !(s.noise_key == key)) {
+        std::shared_ptr<NoiseEntry> e;
+        auto it = g_noise_cache.find(key);
+        if (!force && it != g_noise_cache.end()) e = it->second.lock();
+        if (!e) {
+            e = std::make_shared<NoiseEntry>();
+            std::mt19937 gen(s.sim_seed);
+            std::normal_distribution<double> nd(0, 1);
+            e->noise.resize(s.noise_size);
+            for (long long i = 0; i < s.noise_size; i++) e->noise[i] = nd(gen);
+            e->after = gen;
+            e->d_noise.ensure(sizeof(double) * s.noise_size);
+            check(hipMemcpy(e->d_noise.p, e->noise.data(), sizeof(double) * s.noise_size, hipMemcpyHostToDevice),
+                  "noise H2D");
+            g_noise_cache[key] = e;
+        }
+        s.noise_e = e;
+        s.noise_key = key;
+        s.shuffled_repeats = -1;
+    }
+    if (s.shuffled_repeats == s.noise_repeats) return;
+    std::mt19937 gen = s.noise_e->after;
     s.sh_nodes.assign((size_t)s.noise_repeats * s.nodes, 0);
     s.sh_ts.assign((size_t)s.noise_repeats * s.noise_bw_it, 0);
     std::vector<int> idx(s.nodes);
@@ -269,15 +318,11 @@ void ensure_noise(Session& s, bool force) {
         std::shuffle(tix.begin(), tix.end(), gen);
         std::copy(tix.begin(), tix.end(), s.sh_ts.begin() + (size_t)r * s.noise_bw_it);
     }
-    s.d_noise.ensure(sizeof(double) * s.noise_size);
-    check(hipMemcpy(s.d_noise.p, s.noise.data(), sizeof(double) * s.noise_size, hipMemcpyHostToDevice), "noise H2D");
     s.d_shn.ensure(sizeof(int) * s.sh_nodes.size());
     check(hipMemcpy(s.d_shn.p, s.sh_nodes.data(), sizeof(int) * s.sh_nodes.size(), hipMemcpyHostToDevice), "shuffle H2D");
     s.d_sht.ensure(sizeof(int) * s.sh_ts.size());
     check(hipMemcpy(s.d_sht.p, s.sh_ts.data(), sizeof(int) * s.sh_ts.size(), hipMemcpyHostToDevice), "shuffle H2D");
-    s.last_seed = s.sim_seed; s.last_time_steps = s.time_steps; s.last_nodes = s.nodes;
-    s.last_noise_ts = s.conf.noise_time_steps; s.last_n_noise = n_noise;
-    s.noise_ready = true;
+    s.shuffled_repeats = s.noise_repeats;
 }
 
 void upload_tables(Session& s) {
@@ -477,7 +522,7 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
     a.states_len = s.states_len; a.noise_repeats = s.noise_repeats;
     a.do_delay = s.do_delay; a.dt = s.dt;
     a.SC = io.SC; a.SC_indices = io.SCi; a.gp = io.gp; a.rp = io.rp;
-    a.noise = s.d_noise.as<double>(); a.sh_nodes = s.d_shn.as<int>(); a.sh_ts = s.d_sht.as<int>();
+    a.noise = s.noise_e->d_noise.as<double>(); a.sh_nodes = s.d_shn.as<int>(); a.sh_ts = s.d_sht.as<int>();
     a.bold = io.bold; a.states = io.states; a.gbool = io.gb; a.gint = io.gi;
     a.bw = g_bw;
     a.bw.dt = s.bw_dt; a.bw.dt_itau = a.bw.dt * a.bw.itau;  // core.cpp:291-292
@@ -498,8 +543,8 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
     a.do_delay = any_delay;
     const bool fic_restarts = s.model->id == MODEL_RWW && s.rww.do_fic && s.rww.max_fic_trials > 0;
     bool use_batched = !any_delay && !fic_restarts && !s.conf.states_ts && batched_supported(s.model->id, N);
-    if (s.force_integrator == 0) use_batched = false;
-    if (s.force_integrator == 1 && !use_batched)
+    if (g_force_integrator == 0) use_batched = false;
+    if (g_force_integrator == 1 && !use_batched)
         throw std::runtime_error("batched integrator does not support this configuration (delay, numerical FIC, states_ts or node count)");
     s.last_integrator = use_batched ? 1 : 0;
     int launches = 1;
@@ -507,14 +552,14 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
     if (use_batched) {
         n_groups = build_groups(s, SCi_host);
         int tpw, nt_pad;
-        batched_plan(N, &tpw, &nt_pad);
+        batched_plan(N, s.tpw, &tpw, &nt_pad);
         s.d_afrag.ensure(sizeof(double) * batched_afrag_doubles(N, nt_pad, s.N_SCs));
         check(build_afrag(io.SC, s.N_SCs, N, nt_pad, s.d_afrag.as<double>(), s.stream), "build_afrag launch");
     }
     check(hipEventRecord(s.ev[2], s.stream), "event");
     if (use_batched)
         check(launch_integrate_batched(s.model->id, a, model_consts(s), s.stream, &launches, s.d_groups.as<int>(),
-                                       n_groups, s.d_afrag.as<double>()),
+                                       n_groups, s.d_afrag.as<double>(), s.tpw),
               "integrate_batched launch");
     else check(launch_integrate_general(s.model->id, a, model_consts(s), s.stream), "integrate_general launch");
     check(hipEventRecord(s.ev[3], s.stream), "event");
@@ -529,7 +574,7 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
         f.w_starts = s.d_wins.as<int>(); f.w_ends = s.d_wins.as<int>() + s.n_windows;
         f.wpairs_i = s.d_wpairs.as<int>(); f.wpairs_j = s.d_wpairs.as<int>() + s.n_window_pairs;
         f.fc = io.fc; f.fcd = io.fcd;
-        int path = s.fc_mode;
+        int path = g_fc_mode;
         if (path == 0) path = fc_fcd_fits_lds(f) ? 0 : 1;
         s.last_fc_path = path;
         if (path == 0) {
@@ -566,6 +611,129 @@ void require_session() {
     if (!g_s.initialized || !g_s.model) throw std::runtime_error("cubnm_prepare must be called before cubnm_run");
 }
 
+// core.cpp:194-386: model selection, config, loop counts, tables, noise.
+void prepare_session(Session& s, const char* model_name, int n_config, const char* const* keys,
+                     const char* const* values, int ext_out, int states_ts, int noise_out, int do_delay,
+                     int force_reinit, int use_cpu, int N_SIMS, int nodes, int N_SCs, int time_steps, int BOLD_TR,
+                     int states_sampling, int sim_seed, double dt, double bw_dt, cubnm_dims* dims) {
+    if (!g_bw_ready) { bw_defaults(g_bw); g_bw_ready = true; }
+    if (use_cpu)
+        throw std::runtime_error("use_cpu is not supported: this library is the GPU simulation core "
+                                 "(the CPU restatement is test infrastructure only)");
+    const ModelInfo* m = find_model(model_name);
+    if (!m) throw std::runtime_error(std::string("Model ") + model_name + " not found");
+    if (N_SIMS <= 0 || nodes <= 0 || N_SCs <= 0) throw std::runtime_error("N_SIMS, nodes and N_SCs must be positive");
+    if (nodes > 1024) throw std::runtime_error("nodes > 1024 is not supported");
+    if (dt <= 0 || bw_dt <= 0 || BOLD_TR <= 0 || states_sampling <= 0 || time_steps <= 0)
+        throw std::runtime_error("durations and steps must be positive");
+    const double t0 = now_s();
+    s.model = m;
+    s.N_SIMS = N_SIMS; s.nodes = nodes; s.N_SCs = N_SCs; s.BOLD_TR = BOLD_TR;
+    s.states_sampling = states_sampling; s.time_steps = time_steps; s.do_delay = do_delay != 0;
+    s.sim_seed = sim_seed; s.dt = dt; s.bw_dt = bw_dt / 1000.0; s.noise_out = noise_out != 0;
+    std::vector<std::pair<std::string, std::string>> kv;
+    for (int i = 0; i < n_config; i++) kv.emplace_back(keys[i], values[i]);
+    // rWW config defaults are reset on every call (rww.hpp:61-69)
+    RWW::init_constants(s.rww, dt);
+    apply_config(s, kv);
+    s.conf.ext_out = ext_out != 0;
+    s.conf.states_ts = states_ts != 0;
+    init_model_constants(s);
+    derive_shape(s);
+    ensure_stream(s);
+    ensure_noise(s, force_reinit != 0);
+    upload_tables(s);
+    s.initialized = true;
+    dims->bold_len = s.bold_len; dims->n_pairs = s.n_pairs; dims->n_windows = s.n_windows;
+    dims->n_window_pairs = s.n_window_pairs; dims->states_len = s.states_len;
+    dims->n_state_vars = m->n_state; dims->n_global_params = m->n_global; dims->n_regional_params = m->n_regional;
+    dims->n_global_out_bool = m->n_gb; dims->n_global_out_int = m->n_gi;
+    dims->noise_bw_it = s.noise_bw_it; dims->noise_repeats = s.noise_repeats; dims->bw_it = s.bw_it;
+    dims->inner_it = s.inner_it; dims->n_vols_remove = s.n_vols_remove; dims->corr_len = s.corr_len;
+    dims->do_fc = s.conf.do_fc; dims->do_fcd = s.conf.do_fc && s.conf.do_fcd;
+    dims->modifies_params = s.modifies_params; dims->N_SIMS = N_SIMS; dims->nodes = nodes;
+    dims->noise_size = s.noise_size;
+    dims->init_time = now_s() - t0;
+}
+
+// core.cpp:388-450 on host buffers, first half: host parameter preparation,
+// uploads, kernels and downloads queued on the session's stream.
+void launch_run(Session& s, const double* SC, const int* SC_indices, const double* SC_dist, double* gp, double* rp,
+                const double* v_list, cubnm_outputs* out) {
+    s.t_start = now_s();
+    const int N = s.nodes, S = s.N_SIMS;
+    const ModelInfo* m = s.model;
+    for (int i = 0; i < S; i++)
+        if (SC_indices[i] < 0 || SC_indices[i] >= s.N_SCs) throw std::runtime_error("SC_indices out of range");
+    // host parameter preparation (rWW analytical FIC)
+    s.gb_host.assign((size_t)std::max(1, m->n_gb) * S, 0);
+    const double tp0 = now_s();
+    if (m->id == MODEL_RWW && s.rww.do_fic) analytical_fic(s, SC, SC_indices, gp, rp, s.gb_host.data());
+    s.delays_h.clear(); s.maxd_h.clear();
+    if (s.do_delay) build_delays(s, SC_dist, v_list, s.delays_h, s.maxd_h);
+    s.timings[0] = (now_s() - tp0) * 1e3;
+    const size_t scb = sizeof(double) * (size_t)s.N_SCs * N * N;
+    const size_t gpb = sizeof(double) * (size_t)m->n_global * S;
+    const size_t rpb = sizeof(double) * (size_t)m->n_regional * S * N;
+    const long long st_stride = s.conf.states_ts ? (long long)s.states_len * N : N;
+    const size_t stb = sizeof(double) * (size_t)m->n_state * S * st_stride;
+    s.d_SC.ensure(scb); s.d_SCi.ensure(sizeof(int) * S); s.d_gp.ensure(gpb); s.d_rp.ensure(rpb);
+    s.d_bold.ensure(sizeof(double) * (size_t)S * s.bold_size);
+    s.d_fc.ensure(sizeof(double) * std::max<size_t>(1, (size_t)S * s.n_pairs));
+    s.d_fcd.ensure(sizeof(double) * std::max<size_t>(1, (size_t)S * s.n_window_pairs));
+    s.d_states.ensure(stb);
+    s.d_gb.ensure(std::max<size_t>(1, s.gb_host.size()));
+    s.d_gi.ensure(sizeof(int) * std::max<size_t>(1, (size_t)m->n_gi * S));
+    check(hipEventRecord(s.ev[0], s.stream), "event");
+    check(hipMemcpyAsync(s.d_SC.p, SC, scb, hipMemcpyHostToDevice, s.stream), "SC H2D");
+    check(hipMemcpyAsync(s.d_SCi.p, SC_indices, sizeof(int) * S, hipMemcpyHostToDevice, s.stream), "SC_indices H2D");
+    check(hipMemcpyAsync(s.d_gp.p, gp, gpb, hipMemcpyHostToDevice, s.stream), "params H2D");
+    check(hipMemcpyAsync(s.d_rp.p, rp, rpb, hipMemcpyHostToDevice, s.stream), "params H2D");
+    check(hipMemcpyAsync(s.d_gb.p, s.gb_host.data(), s.gb_host.size(), hipMemcpyHostToDevice, s.stream), "bools H2D");
+    check(hipMemsetAsync(s.d_gi.p, 0, sizeof(int) * std::max<size_t>(1, (size_t)m->n_gi * S), s.stream), "memset");
+    check(hipEventRecord(s.ev[1], s.stream), "event");
+    DevIO io{s.d_SC.as<double>(), s.d_SCi.as<int>(), s.d_gp.as<double>(), s.d_rp.as<double>(),
+             s.d_bold.as<double>(), s.d_fc.as<double>(), s.d_fcd.as<double>(), s.d_states.as<double>(),
+             s.d_gb.as<unsigned char>(), s.d_gi.as<int>()};
+    run_kernels(s, io, s.do_delay ? &s.delays_h : nullptr, s.do_delay ? &s.maxd_h : nullptr, SC_indices);
+    check(hipMemcpyAsync(out->sim_bold, s.d_bold.p, sizeof(double) * (size_t)S * s.bold_size, hipMemcpyDeviceToHost, s.stream), "BOLD D2H");
+    if (s.conf.do_fc && out->sim_fc_trils)
+        check(hipMemcpyAsync(out->sim_fc_trils, s.d_fc.p, sizeof(double) * (size_t)S * s.n_pairs, hipMemcpyDeviceToHost, s.stream), "FC D2H");
+    if (s.conf.do_fc && s.conf.do_fcd && out->sim_fcd_trils)
+        check(hipMemcpyAsync(out->sim_fcd_trils, s.d_fcd.p, sizeof(double) * (size_t)S * s.n_window_pairs, hipMemcpyDeviceToHost, s.stream), "FCD D2H");
+    if (out->sim_states) check(hipMemcpyAsync(out->sim_states, s.d_states.p, stb, hipMemcpyDeviceToHost, s.stream), "states D2H");
+    if (out->global_bools && m->n_gb > 0)
+        check(hipMemcpyAsync(out->global_bools, s.d_gb.p, (size_t)m->n_gb * S, hipMemcpyDeviceToHost, s.stream), "bools D2H");
+    if (out->global_ints && m->n_gi > 0)
+        check(hipMemcpyAsync(out->global_ints, s.d_gi.p, sizeof(int) * (size_t)m->n_gi * S, hipMemcpyDeviceToHost, s.stream), "ints D2H");
+    if (s.modifies_params) {
+        check(hipMemcpyAsync(gp, s.d_gp.p, gpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
+        check(hipMemcpyAsync(rp, s.d_rp.p, rpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
+    }
+    check(hipEventRecord(s.ev[5], s.stream), "event");
+}
+
+// Second half: wait for the stream, timings, host-side noise outputs.
+void finish_run(Session& s, cubnm_outputs* out) {
+    check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+    s.timings[1] = ev_ms(s.ev[0], s.ev[1]);
+    s.timings[2] = ev_ms(s.ev[2], s.ev[3]);
+    s.timings[3] = ev_ms(s.ev[3], s.ev[4]);
+    s.timings[4] = ev_ms(s.ev[4], s.ev[5]);
+    if (s.noise_out) {
+        if (out->noise) std::copy(s.noise_e->noise.begin(), s.noise_e->noise.end(), out->noise);
+        if (out->shuffled_nodes) std::copy(s.sh_nodes.begin(), s.sh_nodes.end(), out->shuffled_nodes);
+        if (out->shuffled_ts) std::copy(s.sh_ts.begin(), s.sh_ts.end(), out->shuffled_ts);
+    }
+    out->run_time = now_s() - s.t_start;
+}
+
+// Relative cost of a group for launch ordering: node-steps times the
+// coupling width.
+double group_cost(const cubnm_group& g) {
+    return (double)g.N_SIMS * g.nodes * ((double)g.time_steps / std::max(1e-9, g.dt)) * (2.0 * g.nodes + 42.0);
+}
+
 }  // namespace
 
 void set_last_error(const std::string& e) { g_err = e; }
@@ -593,7 +761,7 @@ int cubnm_set_const(const char* key, double value) {
     return 0;
 }
 
-int cubnm_force_integrator(int force) { g_s.force_integrator = force; return 0; }
+int cubnm_force_integrator(int force) { g_force_integrator = force; return 0; }
 
 int cubnm_set_tuning(const char* key, int value) {
     if (std::strcmp(key, "batched_tpw") == 0) {
@@ -603,7 +771,7 @@ int cubnm_set_tuning(const char* key, int value) {
     }
     if (std::strcmp(key, "fc_mode") == 0) {
         if (value < 0 || value > 2) { g_err = "fc_mode must be 0 (auto), 1 (FP64 Gram) or 2 (split-BF16 Gram)"; return -1; }
-        g_s.fc_mode = value;
+        g_fc_mode = value;
         return 0;
     }
     g_err = std::string("unknown tuning key ") + key;
@@ -622,47 +790,9 @@ int cubnm_prepare(const char* model_name, int n_config, const char* const* keys,
                   int N_SIMS, int nodes, int N_SCs, int time_steps, int BOLD_TR, int states_sampling,
                   int sim_seed, double dt, double bw_dt, cubnm_dims* dims) {
     try {
-        if (!g_bw_ready) cubnm_init();
-        if (use_cpu)
-            throw std::runtime_error("use_cpu is not supported: this library is the GPU simulation core "
-                                     "(the CPU restatement is test infrastructure only)");
-        const ModelInfo* m = find_model(model_name);
-        if (!m) throw std::runtime_error(std::string("Model ") + model_name + " not found");
-        if (N_SIMS <= 0 || nodes <= 0 || N_SCs <= 0) throw std::runtime_error("N_SIMS, nodes and N_SCs must be positive");
-        if (nodes > 1024) throw std::runtime_error("nodes > 1024 is not supported");
-        if (dt <= 0 || bw_dt <= 0 || BOLD_TR <= 0 || states_sampling <= 0 || time_steps <= 0)
-            throw std::runtime_error("durations and steps must be positive");
-        Session& s = g_s;
-        const double t0 = now_s();
-        const bool new_model = (s.model != m) || !s.initialized;
-        if (new_model) { s.noise_ready = false; }
-        s.model = m;
-        s.N_SIMS = N_SIMS; s.nodes = nodes; s.N_SCs = N_SCs; s.BOLD_TR = BOLD_TR;
-        s.states_sampling = states_sampling; s.time_steps = time_steps; s.do_delay = do_delay != 0;
-        s.sim_seed = sim_seed; s.dt = dt; s.bw_dt = bw_dt / 1000.0; s.noise_out = noise_out != 0;
-        std::vector<std::pair<std::string, std::string>> kv;
-        for (int i = 0; i < n_config; i++) kv.emplace_back(keys[i], values[i]);
-        // rWW config defaults are reset on every call (rww.hpp:61-69)
-        RWW::init_constants(s.rww, dt);
-        apply_config(s, kv);
-        s.conf.ext_out = ext_out != 0;
-        s.conf.states_ts = states_ts != 0;
-        init_model_constants(s);
-        derive_shape(s);
-        ensure_stream(s);
-        ensure_noise(s, force_reinit != 0);
-        upload_tables(s);
-        s.initialized = true;
-        dims->bold_len = s.bold_len; dims->n_pairs = s.n_pairs; dims->n_windows = s.n_windows;
-        dims->n_window_pairs = s.n_window_pairs; dims->states_len = s.states_len;
-        dims->n_state_vars = m->n_state; dims->n_global_params = m->n_global; dims->n_regional_params = m->n_regional;
-        dims->n_global_out_bool = m->n_gb; dims->n_global_out_int = m->n_gi;
-        dims->noise_bw_it = s.noise_bw_it; dims->noise_repeats = s.noise_repeats; dims->bw_it = s.bw_it;
-        dims->inner_it = s.inner_it; dims->n_vols_remove = s.n_vols_remove; dims->corr_len = s.corr_len;
-        dims->do_fc = s.conf.do_fc; dims->do_fcd = s.conf.do_fc && s.conf.do_fcd;
-        dims->modifies_params = s.modifies_params; dims->N_SIMS = N_SIMS; dims->nodes = nodes;
-        dims->noise_size = s.noise_size;
-        dims->init_time = now_s() - t0;
+        prepare_session(g_s, model_name, n_config, keys, values, ext_out, states_ts, noise_out, do_delay,
+                        force_reinit, use_cpu, N_SIMS, nodes, N_SCs, time_steps, BOLD_TR, states_sampling, sim_seed,
+                        dt, bw_dt, dims);
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();
@@ -674,74 +804,75 @@ int cubnm_run(const double* SC, const int* SC_indices, const double* SC_dist, do
               const double* v_list, cubnm_outputs* out) {
     try {
         require_session();
-        Session& s = g_s;
-        const double t_start = now_s();
-        const int N = s.nodes, S = s.N_SIMS;
-        const ModelInfo* m = s.model;
-        for (int i = 0; i < S; i++)
-            if (SC_indices[i] < 0 || SC_indices[i] >= s.N_SCs) throw std::runtime_error("SC_indices out of range");
-        // host parameter preparation (rWW analytical FIC)
-        std::vector<unsigned char> gb((size_t)std::max(1, m->n_gb) * S, 0);
-        const double tp0 = now_s();
-        if (m->id == MODEL_RWW && s.rww.do_fic) analytical_fic(s, SC, SC_indices, gp, rp, gb.data());
-        std::vector<int> delays, maxd;
-        if (s.do_delay) build_delays(s, SC_dist, v_list, delays, maxd);
-        s.timings[0] = (now_s() - tp0) * 1e3;
-        // H2D
-        const size_t scb = sizeof(double) * (size_t)s.N_SCs * N * N;
-        const size_t gpb = sizeof(double) * (size_t)m->n_global * S;
-        const size_t rpb = sizeof(double) * (size_t)m->n_regional * S * N;
-        const long long st_stride = s.conf.states_ts ? (long long)s.states_len * N : N;
-        const size_t stb = sizeof(double) * (size_t)m->n_state * S * st_stride;
-        s.d_SC.ensure(scb); s.d_SCi.ensure(sizeof(int) * S); s.d_gp.ensure(gpb); s.d_rp.ensure(rpb);
-        s.d_bold.ensure(sizeof(double) * (size_t)S * s.bold_size);
-        s.d_fc.ensure(sizeof(double) * std::max<size_t>(1, (size_t)S * s.n_pairs));
-        s.d_fcd.ensure(sizeof(double) * std::max<size_t>(1, (size_t)S * s.n_window_pairs));
-        s.d_states.ensure(stb);
-        s.d_gb.ensure(std::max<size_t>(1, gb.size()));
-        s.d_gi.ensure(sizeof(int) * std::max<size_t>(1, (size_t)m->n_gi * S));
-        check(hipEventRecord(s.ev[0], s.stream), "event");
-        check(hipMemcpyAsync(s.d_SC.p, SC, scb, hipMemcpyHostToDevice, s.stream), "SC H2D");
-        check(hipMemcpyAsync(s.d_SCi.p, SC_indices, sizeof(int) * S, hipMemcpyHostToDevice, s.stream), "SC_indices H2D");
-        check(hipMemcpyAsync(s.d_gp.p, gp, gpb, hipMemcpyHostToDevice, s.stream), "params H2D");
-        check(hipMemcpyAsync(s.d_rp.p, rp, rpb, hipMemcpyHostToDevice, s.stream), "params H2D");
-        check(hipMemcpyAsync(s.d_gb.p, gb.data(), gb.size(), hipMemcpyHostToDevice, s.stream), "bools H2D");
-        check(hipMemsetAsync(s.d_gi.p, 0, sizeof(int) * std::max<size_t>(1, (size_t)m->n_gi * S), s.stream), "memset");
-        check(hipEventRecord(s.ev[1], s.stream), "event");
-        DevIO io{s.d_SC.as<double>(), s.d_SCi.as<int>(), s.d_gp.as<double>(), s.d_rp.as<double>(),
-                 s.d_bold.as<double>(), s.d_fc.as<double>(), s.d_fcd.as<double>(), s.d_states.as<double>(),
-                 s.d_gb.as<unsigned char>(), s.d_gi.as<int>()};
-        run_kernels(s, io, s.do_delay ? &delays : nullptr, s.do_delay ? &maxd : nullptr, SC_indices);
-        // D2H
-        check(hipMemcpyAsync(out->sim_bold, s.d_bold.p, sizeof(double) * (size_t)S * s.bold_size, hipMemcpyDeviceToHost, s.stream), "BOLD D2H");
-        if (s.conf.do_fc && out->sim_fc_trils)
-            check(hipMemcpyAsync(out->sim_fc_trils, s.d_fc.p, sizeof(double) * (size_t)S * s.n_pairs, hipMemcpyDeviceToHost, s.stream), "FC D2H");
-        if (s.conf.do_fc && s.conf.do_fcd && out->sim_fcd_trils)
-            check(hipMemcpyAsync(out->sim_fcd_trils, s.d_fcd.p, sizeof(double) * (size_t)S * s.n_window_pairs, hipMemcpyDeviceToHost, s.stream), "FCD D2H");
-        if (out->sim_states) check(hipMemcpyAsync(out->sim_states, s.d_states.p, stb, hipMemcpyDeviceToHost, s.stream), "states D2H");
-        if (out->global_bools && m->n_gb > 0)
-            check(hipMemcpyAsync(out->global_bools, s.d_gb.p, (size_t)m->n_gb * S, hipMemcpyDeviceToHost, s.stream), "bools D2H");
-        if (out->global_ints && m->n_gi > 0)
-            check(hipMemcpyAsync(out->global_ints, s.d_gi.p, sizeof(int) * (size_t)m->n_gi * S, hipMemcpyDeviceToHost, s.stream), "ints D2H");
-        if (s.modifies_params) {
-            check(hipMemcpyAsync(gp, s.d_gp.p, gpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
-            check(hipMemcpyAsync(rp, s.d_rp.p, rpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
-        }
-        check(hipEventRecord(s.ev[5], s.stream), "event");
-        check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
-        s.timings[1] = ev_ms(s.ev[0], s.ev[1]);
-        s.timings[2] = ev_ms(s.ev[2], s.ev[3]);
-        s.timings[3] = ev_ms(s.ev[3], s.ev[4]);
-        s.timings[4] = ev_ms(s.ev[4], s.ev[5]);
-        if (s.noise_out) {
-            if (out->noise) std::copy(s.noise.begin(), s.noise.end(), out->noise);
-            if (out->shuffled_nodes) std::copy(s.sh_nodes.begin(), s.sh_nodes.end(), out->shuffled_nodes);
-            if (out->shuffled_ts) std::copy(s.sh_ts.begin(), s.sh_ts.end(), out->shuffled_ts);
-        }
-        out->run_time = now_s() - t_start;
+        launch_run(g_s, SC, SC_indices, SC_dist, gp, rp, v_list, out);
+        finish_run(g_s, out);
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();
+        if (g_s.stream) hipStreamSynchronize(g_s.stream);
+        return -1;
+    }
+}
+
+int cubnm_prepare_groups(int n_groups, cubnm_group* groups) {
+    try {
+        if (n_groups <= 0) throw std::runtime_error("need at least one group");
+        while ((int)g_pool.size() < n_groups) g_pool.emplace_back(new Session());
+        for (int i = 0; i < n_groups; i++) {
+            cubnm_group& g = groups[i];
+            try {
+                prepare_session(*g_pool[i], g.model_name, g.n_config, g.config_keys, g.config_values, g.ext_out,
+                                g.states_ts, g.noise_out, g.do_delay, g.force_reinit, 0, g.N_SIMS, g.nodes, g.N_SCs,
+                                g.time_steps, g.BOLD_TR, g.states_sampling, g.sim_seed, g.dt, g.bw_dt, &g.dims);
+                g_pool[i]->tpw = g.batched_tpw;
+            } catch (const std::exception& e) {
+                throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int cubnm_run_groups(int n_groups, cubnm_group* groups) {
+    int launched_upto = 0;
+    std::vector<int> order(std::max(0, n_groups));
+    try {
+        if (n_groups <= 0 || (int)g_pool.size() < n_groups) throw std::runtime_error("cubnm_prepare_groups first");
+        for (int i = 0; i < n_groups; i++)
+            if (!g_pool[i]->initialized || g_pool[i]->N_SIMS != groups[i].N_SIMS || g_pool[i]->nodes != groups[i].nodes)
+                throw std::runtime_error("group " + std::to_string(i) + " does not match its prepared session");
+        // costliest groups first: their kernels start while the host prepares the rest
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return group_cost(groups[a]) > group_cost(groups[b]); });
+        for (; launched_upto < n_groups; launched_upto++) {
+            const int i = order[launched_upto];
+            cubnm_group& g = groups[i];
+            try {
+                launch_run(*g_pool[i], g.SC, g.SC_indices, g.SC_dist, g.global_params, g.regional_params, g.v_list,
+                           &g.out);
+            } catch (const std::exception& e) {
+                throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
+            }
+        }
+        for (int k = 0; k < n_groups; k++) {
+            const int i = order[k];
+            try {
+                finish_run(*g_pool[i], &groups[i].out);
+            } catch (const std::exception& e) {
+                throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        for (int k = 0; k < launched_upto + 1 && k < n_groups; k++) {
+            Session& s = *g_pool[order[k]];
+            if (s.stream) hipStreamSynchronize(s.stream);
+        }
         return -1;
     }
 }

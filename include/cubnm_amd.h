@@ -91,6 +91,42 @@ int cubnm_run_device(const double* d_SC, const int* d_SC_indices, const double* 
                      double* d_global_params, double* d_regional_params, const double* d_v_list,
                      cubnm_outputs* d_out);
 
+/* Heterogeneous batches (no counterpart in the reference, whose
+ * run_simulations takes one model, node count and duration per call): many
+ * groups, each a complete run_simulations request (its own model, node
+ * count, duration, config, SCs and parameters), run in one call.  Each group
+ * gets its own session and stream; all groups' kernels are queued before any
+ * is waited for, costliest first, so they share the GPU.  Host buffers, with
+ * the same layouts and in-place parameter write-back as cubnm_run.  Sessions
+ * persist between calls (group i reuses session i, and its noise when the
+ * seed, node count and segment agree).  Separate from the single-request
+ * session behind cubnm_prepare / cubnm_run. */
+typedef struct cubnm_group {
+    /* request, as cubnm_prepare */
+    const char* model_name;
+    int n_config;
+    const char* const* config_keys;
+    const char* const* config_values;
+    int ext_out, states_ts, noise_out, do_delay, force_reinit;
+    int N_SIMS, nodes, N_SCs, time_steps, BOLD_TR, states_sampling, sim_seed;
+    double dt, bw_dt;
+    int batched_tpw;              /* launch shape for this group (0 = auto) */
+    /* inputs, as cubnm_run */
+    const double* SC;
+    const int* SC_indices;
+    const double* SC_dist;
+    double* global_params;
+    double* regional_params;
+    const double* v_list;
+    /* output sizes, filled by cubnm_prepare_groups */
+    cubnm_dims dims;
+    /* outputs, allocated by the caller from dims */
+    cubnm_outputs out;
+} cubnm_group;
+
+int cubnm_prepare_groups(int n_groups, cubnm_group* groups);
+int cubnm_run_groups(int n_groups, cubnm_group* groups);
+
 /* Timings of the last run (milliseconds): [0] parameter prep on host,
  * [1] host->device copies, [2] integration kernel, [3] FC/FCD kernel,
  * [4] device->host copies; [5] integration kernel launches. */

@@ -120,3 +120,16 @@ def test_sc_indices_beyond_scs_raise():
     a[2] = np.array([0, 3], np.intc)
     with pytest.raises(ValueError, match="max\\(SC_indices\\)"):
         core.run_simulations(*a)
+
+
+def test_run_groups_argument_errors():
+    import cubnm_amd.core as core
+    assert core.run_groups([]) == []
+    with pytest.raises(TypeError):
+        core.run_groups(5)
+    with pytest.raises(TypeError):
+        core.run_groups([[1, 2]])
+    a = _args()
+    a[13] = True
+    with pytest.raises(RuntimeError, match="use_cpu"):
+        core.run_groups([tuple(a)])

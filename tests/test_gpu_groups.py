@@ -1,0 +1,58 @@
+"""Heterogeneous batches through one call (cubnm_run_groups / core.run_groups).
+
+Each request of a mixed batch (different models, node counts, durations,
+SCs, FIC settings and noise outputs) must give exactly what the same request
+gives alone through run_simulations (same kernels, same noise), which the
+parity tests tie to the oracle; one request is also checked against the
+oracle directly at the usual 1e-9 tolerance.
+"""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    reqs = []
+    specs = [("rWW", 10, 12, 3, 1), ("JR", 17, 8, 2, 2), ("Kuramoto", 6, 6, 4, 3), ("WC", 33, 10, 5, 4),
+             ("rWW", 10, 20, 2, 5), ("rWW", 70, 8, 17, 6)]
+    for model, N, dur, S, seed in specs:
+        sc = cases.synthetic_sc(N, seed=seed)
+        gp, rp = cases.params(model, S, N, G=np.linspace(0.3, 1.5, S))
+        cfg = cases.model_config(model, window_size=4, window_step=2, bold_remove_s=2)
+        reqs.append(cases.sim_args(model, sc, gp, rp, cfg, S, duration_s=dur, noise_out=True, sim_seed=seed % 3))
+    return reqs
+
+
+def test_groups_equal_single_requests():
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    reqs = _requests()
+    grouped = _core.core.run_groups([cases.copy_args(r) for r in reqs])
+    assert len(grouped) == len(reqs)
+    for r, g in zip(reqs, grouped):
+        a = cases.copy_args(r)
+        single = _core.run_simulations(*a)
+        assert set(g) == set(single)
+        for k in single:
+            if k in ("init_time", "run_time"):
+                continue
+            np.testing.assert_array_equal(g[k], single[k], err_msg=f"{r[0]} N={r[15]}: {k}")
+
+
+def test_groups_match_oracle_and_write_back():
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    reqs = _requests()[:2]
+    a = [cases.copy_args(r) for r in reqs]
+    grouped = _core.core.run_groups(a)
+    ref = cases.run_oracle(cases.copy_args(reqs[0]))
+    assert cases.bold_rel_err(grouped[0]["sim_bold"], ref["sim_bold"]).max() <= 1e-9
+    assert cases.max_abs_err(grouped[0]["sim_fc_trils"], ref["sim_fc_trils"]) <= 1e-9
+    # analytical FIC wrote wIE into the request's own regional_params
+    b = cases.copy_args(reqs[0])
+    cases.run_oracle(b)
+    np.testing.assert_allclose(a[0][5], b[5], rtol=1e-12, atol=0)
+    assert np.all(a[0][5][2] > 0)
