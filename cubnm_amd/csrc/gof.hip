@@ -111,16 +111,17 @@ __global__ void __launch_bounds__(GOF_THREADS) fcd_ks_kernel(const double* sim, 
             }
         }
         __syncthreads();
-        double d = 0.0;
-        const double inv_n = 1.0 / n, inv_m = 1.0 / m;
+        // |F_sim - F_emp| = |c_s m - c_e n| / (n m): the integer numerator is
+        // exact, so equal ECDFs give exactly 0 and the one division rounds once
+        long long dmax = 0;
         for (int i = threadIdx.x; i < n + m; i += blockDim.x) {
             const double z = i < n ? v[i] : emp_sorted[i - n];
-            const double fs = upper_bound(v, n, z) * inv_n;
-            const double fe = upper_bound(emp_sorted, m, z) * inv_m;
-            d = fmax(d, fabs(fs - fe));
+            const long long cs = upper_bound(v, n, z), ce = upper_bound(emp_sorted, m, z);
+            const long long dd = cs * m - ce * n;
+            dmax = max(dmax, dd < 0 ? -dd : dd);
         }
-        d = wg_max(d, red);
-        if (threadIdx.x == 0) out[s] = nan_seen > 0.0 ? NAN : d;
+        const double d = wg_max((double)dmax, red);   // integers below 2^53: exact in double
+        if (threadIdx.x == 0) out[s] = nan_seen > 0.0 ? NAN : d / ((double)n * (double)m);
         __syncthreads();
     }
 }
