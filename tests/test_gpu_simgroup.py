@@ -1,7 +1,12 @@
 """SimGroup layer end to end on the GPU core: the same group run through the
 HIP core and through the oracle gives the same outputs (BOLD norm-wise
 relative <= 1e-9, FC/FCD absolute <= 1e-9, FIC wIE relative <= 1e-12), and
-score() on the GPU equals score(force_cpu=True) (SciPy)."""
+score() on the GPU equals score(force_cpu=True) (SciPy).
+
+The G range keeps every simulation well conditioned: moving G by one ulp
+changes its BOLD by < 1e-11 relative in the oracle (WC near G = 1.72 on this
+SC is not: one ulp moves its BOLD by 5e-3, so no FP64 reimplementation can
+match there at 1e-9)."""
 import numpy as np
 import pytest
 
@@ -14,7 +19,7 @@ def _group(model_cls, **kw):
     sg = model_cls(duration=20, TR=1, sc=cases.synthetic_sc(24, seed=5), window_size=4, window_step=2,
                    bold_remove_s=4, **kw)
     sg.N = 20
-    sg.param_lists["G"] = np.linspace(0.2, 2.0, 20)
+    sg.param_lists["G"] = np.linspace(0.2, 1.5, 20)
     sg._set_default_params()
     return sg
 
