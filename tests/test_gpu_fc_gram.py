@@ -5,8 +5,9 @@ The expected values restate the reference's definitions (fc.cpp:1-91):
 Pearson r over the post-burn-in volumes for every pair (i > j, row-major,
 optionally same-hemisphere only), Pearson r inside each window, and Pearson
 r between window-FC vectors; the window table is the oracle's (pinned by the
-reference fixtures).  Tolerances: FP64 Gram 1e-12 absolute (summation order
-only); split-BF16 Gram, stated per result and asserted here:
+reference fixtures).  Tolerances: FP64 Gram 1e-10 absolute (summation order
+only: NumPy and the MFMA K loop add the 79,800 pair products of an N=400 FCD
+entry in different orders; observed 4e-12); split-BF16 Gram, stated per result and asserted here:
 FC 1e-4, FCD 1e-4 absolute (operands carry ~2^-16 relative error; each
 entry averages it over K products; measured errors are printed).
 """
@@ -64,8 +65,8 @@ def test_fp64_gram_matches_numpy(core, nodes, exc_inter):
     bold = out["sim_bold"].reshape(3, d["bold_len"], nodes)
     for s in range(3):
         fc, fcd = _expected(bold[s], d["n_vols_remove"], d["window_starts"], d["window_ends"], exc_inter)
-        np.testing.assert_allclose(out["sim_fc_trils"][s], fc, rtol=0, atol=1e-12)
-        np.testing.assert_allclose(out["sim_fcd_trils"][s], fcd, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(out["sim_fc_trils"][s], fc, rtol=0, atol=1e-10)
+        np.testing.assert_allclose(out["sim_fcd_trils"][s], fcd, rtol=0, atol=1e-10)
 
 
 def test_fp64_gram_equals_lds_kernel(core):
