@@ -19,12 +19,16 @@
 // Two arithmetic forms of the Gram product:
 //   * FP64: v_mfma_f64_16x16x4f64 with FP64 accumulation (default; parity
 //     with the FP64 oracle at the 1e-13 level);
-//   * split-BF16: each FP64 operand x is split into hi = bf16(x) and
-//     lo = bf16(x - hi); x.y ~ hi.hi + hi.lo + lo.hi on
-//     v_mfma_f32_16x16x32_bf16 with FP32 accumulation, flushed to FP64
-//     every 512 K.  The normalisations stay FP64.  Relative operand error is
-//     about 2^-16, so an entry's error is about 2^-16 / sqrt(K) for
-//     unit-norm rows (stated per result by the tests).
+//   * split-BF16 (for the static FC and the FCD, the large-K products):
+//     each FP64 operand x is split into hi = bf16(x) and lo = bf16(x - hi);
+//     x.y ~ hi.hi + hi.lo + lo.hi on v_mfma_f32_16x16x32_bf16 with FP32
+//     accumulation, flushed to FP64 every 512 K.  The normalisations stay
+//     FP64, and so do the window FCs (K = one window, a small share of the
+//     work): the FCD is a Pearson r between window-FC vectors whose spread
+//     across pairs can be small, so their errors are amplified by
+//     1 / (their standard deviation) and must stay at FP64 level.  Operand
+//     error is about 2^-16 relative; measured errors are stated per result
+//     by the tests.
 //
 // Tiles: 64 x 64 outputs per workgroup of 4 waves (each 32 x 32 = 2 x 2
 // MFMA blocks); only tiles on or below the diagonal are launched.
@@ -87,12 +91,11 @@ __device__ inline long long pair_index(int i, int j, const int* pair_of, int row
 // ---------------------------------------------------------------- 1. z-scores
 // blockIdx.x = simulation within the batch, blockIdx.y = row range (0 = FC,
 // 1 + w = window w).  Output row stride is k_fc or k_w.
-template <bool BF16>
+template <bool FC_BF16>
 __global__ void __launch_bounds__(ZT) zscore_kernel(const double* bold, int bold_len, int N, int r0_fc,
                                                     const int* w_starts, const int* w_ends, int n_windows,
                                                     int k_fc, int k_w, double* x_fc, double* x_w,
-                                                    unsigned short* h_fc, unsigned short* l_fc,
-                                                    unsigned short* h_w, unsigned short* l_w) {
+                                                    unsigned short* h_fc, unsigned short* l_fc) {
     const int b = blockIdx.x, m = blockIdx.y;
     const double* B = bold + (long long)b * bold_len * N;
     int r0, r1, K;
@@ -113,12 +116,8 @@ __global__ void __launch_bounds__(ZT) zscore_kernel(const double* bold, int bold
         const long long row = base + (long long)i * K;
         for (int k = 0; k < K; k++) {
             const double z = k < len ? (B[(long long)(r0 + k) * N + i] - mean) * inv : 0.0;
-            if (BF16) {
-                if (m == 0) split_store(z, h_fc, l_fc, row + k);
-                else split_store(z, h_w, l_w, row + k);
-            } else {
-                (m == 0 ? x_fc : x_w)[row + k] = z;
-            }
+            if (FC_BF16 && m == 0) split_store(z, h_fc, l_fc, row + k);
+            else (m == 0 ? x_fc : x_w)[row + k] = z;
         }
     }
 }
@@ -276,10 +275,10 @@ GramLayout gram_layout(const FcArgs& a, int max_win_len, bool bf16) {
     GramLayout g{};
     const int step = bf16 ? KCB : KC64;
     g.k_fc = round_up(std::max(1, a.bold_len - a.n_vols_remove), step);
-    g.k_w = round_up(std::max(1, max_win_len), step);
+    g.k_w = round_up(std::max(1, max_win_len), KC64);       // window FCs are FP64 in both forms
     g.p_ld = round_up(std::max(1, a.n_pairs), step);
-    const size_t e = bf16 ? 4 : 8;  // bytes per element (hi + lo, or one double)
-    size_t per = e * (size_t)a.nodes * g.k_fc + e * (size_t)a.n_windows * a.nodes * g.k_w;
+    const size_t e = bf16 ? 4 : 8;  // bytes per FC operand element (hi + lo, or one double)
+    size_t per = e * (size_t)a.nodes * g.k_fc + 8 * (size_t)a.n_windows * a.nodes * g.k_w;
     per += 8 * (size_t)a.n_windows * g.p_ld;                 // FP64 window-FC vectors
     if (bf16) per += 4 * (size_t)a.n_windows * g.p_ld;       // their hi/lo split
     g.per_sim_bytes = per;
@@ -305,14 +304,16 @@ hipError_t launch_fc_gram(const FcArgs& a, int max_win_len, int split_bf16, cons
     const size_t n_fc = (size_t)batch * N * g.k_fc, n_w = (size_t)batch * W * N * g.k_w;
     const size_t n_u = (size_t)batch * W * g.p_ld;
     double *x_fc = nullptr, *x_w = nullptr;
-    unsigned short *h_fc = nullptr, *l_fc = nullptr, *h_w = nullptr, *l_w = nullptr, *uh = nullptr, *ul = nullptr;
+    unsigned short *h_fc = nullptr, *l_fc = nullptr, *uh = nullptr, *ul = nullptr;
     if (bf16) {
-        h_fc = (unsigned short*)w; l_fc = h_fc + n_fc; h_w = l_fc + n_fc; l_w = h_w + n_w;
-        w += 2 * e * (n_fc + n_w);
+        h_fc = (unsigned short*)w; l_fc = h_fc + n_fc;
+        w += 2 * e * n_fc;
     } else {
-        x_fc = (double*)w; x_w = x_fc + n_fc;
-        w += e * (n_fc + n_w);
+        x_fc = (double*)w;
+        w += e * n_fc;
     }
+    x_w = (double*)w;
+    w += 8 * n_w;
     double* U = (double*)w;
     w += 8 * n_u;
     if (bf16) { uh = (unsigned short*)w; ul = uh + n_u; }
@@ -326,12 +327,11 @@ hipError_t launch_fc_gram(const FcArgs& a, int max_win_len, int split_bf16, cons
         const double* bold = a.bold + (long long)s0 * a.bold_len * N;
         if (bf16)
             hipLaunchKernelGGL(zscore_kernel<true>, dim3(nb, mats), dim3(ZT), 0, s, bold, a.bold_len, N,
-                               a.n_vols_remove, a.w_starts, a.w_ends, W, g.k_fc, g.k_w, nullptr, nullptr,
-                               h_fc, l_fc, h_w, l_w);
+                               a.n_vols_remove, a.w_starts, a.w_ends, W, g.k_fc, g.k_w, nullptr, x_w, h_fc, l_fc);
         else
             hipLaunchKernelGGL(zscore_kernel<false>, dim3(nb, mats), dim3(ZT), 0, s, bold, a.bold_len, N,
                                a.n_vols_remove, a.w_starts, a.w_ends, W, g.k_fc, g.k_w, x_fc, x_w,
-                               nullptr, nullptr, nullptr, nullptr);
+                               nullptr, nullptr);
         // static FC straight into the output
         double* fc = a.fc + (long long)s0 * a.n_pairs;
         const int tn = tri_tiles(N), tw = tri_tiles(W);
@@ -342,12 +342,8 @@ hipError_t launch_fc_gram(const FcArgs& a, int max_win_len, int split_bf16, cons
             hipLaunchKernelGGL(gram_f64_kernel, dim3(tn * nb), dim3(256), 0, s, x_fc,
                                (long long)N * g.k_fc, N, g.k_fc, fc, (long long)a.n_pairs, pair_of, tn);
         if (mats > 1) {
-            if (bf16)
-                hipLaunchKernelGGL(gram_bf16x3_kernel, dim3(tn * nb * W), dim3(256), 0, s, h_w, l_w,
-                                   (long long)N * g.k_w, N, g.k_w, U, (long long)g.p_ld, pair_of, tn);
-            else
-                hipLaunchKernelGGL(gram_f64_kernel, dim3(tn * nb * W), dim3(256), 0, s, x_w,
-                                   (long long)N * g.k_w, N, g.k_w, U, (long long)g.p_ld, pair_of, tn);
+            hipLaunchKernelGGL(gram_f64_kernel, dim3(tn * nb * W), dim3(256), 0, s, x_w,
+                               (long long)N * g.k_w, N, g.k_w, U, (long long)g.p_ld, pair_of, tn);
             double* fcd = a.fcd + (long long)s0 * a.n_window_pairs;
             if (bf16) {
                 hipLaunchKernelGGL(center_kernel<true>, dim3(nb * W), dim3(ZT), 0, s, U, a.n_pairs, g.p_ld, uh, ul);
