@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 def _requests():
     reqs = []
-    specs = [("rWW", 10, 12, 3, 1), ("JR", 17, 8, 2, 2), ("Kuramoto", 6, 6, 4, 3), ("WC", 33, 10, 5, 4),
+    specs = [("rWW", 10, 12, 3, 1), ("JR", 17, 8, 2, 2), ("Kuramoto", 6, 10, 4, 3), ("WC", 33, 10, 5, 4),
              ("rWW", 10, 20, 2, 5), ("rWW", 70, 8, 17, 6)]
     for model, N, dur, S, seed in specs:
         sc = cases.synthetic_sc(N, seed=seed)
