@@ -135,6 +135,12 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                     n0[sl] = p[0];
                     n1[sl] = (M::n_noise > 1) ? p[1] : 0.0;
                 }
+                // the part of the model step that does not read this step's
+                // coupling input, placed before the product so that the
+                // scheduler can issue it between the MFMAs
+                double tp[NS][M::n_tmp];
+#pragma unroll
+                for (int sl = 0; sl < NS; sl++) M::pre(mc, s[sl], gp, rp[sl], n0[sl], n1[sl], tp[sl]);
                 const double* xb = xs + buf * buf_stride;
                 v4d acc[TPW][NX];
 #pragma unroll
@@ -196,7 +202,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                         double gin;
                         if constexpr (M::is_osc) gin = cs[sl] * acc[t][0][r] - sn[sl] * acc[t][1][r];
                         else gin = acc[t][0][r];
-                        M::step(mc, s[sl], gp, rp[sl], gin, n0[sl], n1[sl]);
+                        M::post(mc, s[sl], gp, rp[sl], gin, n0[sl], n1[sl], tp[sl]);
                         if (j < N) {
                             publish<M>(xn, KN, j, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
                             if (last)

@@ -71,6 +71,35 @@ struct RWW {
         s[4] = fmax(0.0, fmin(1.0, s[4]));
         s[5] = fmax(0.0, fmin(1.0, s[5]));
     }
+    // The same step in two parts for the batched integrator: `pre` is
+    // everything that does not read this step's coupling input g (the
+    // inhibitory population, and the partial sums of I_E), so it can run
+    // while the coupling product is on the matrix cores; `post` finishes
+    // I_E and the excitatory population.  Operations and their order are
+    // those of `step`.
+    static constexpr int n_tmp = 2;
+    CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double n1,
+                             double* t) {
+        (void)gp; (void)n0;
+        t[0] = c.w_E__I_0 + rp[0] * rp[1] * s[4];
+        t[1] = rp[2] * s[5];
+        s[1] = c.w_I__I_0 + rp[1] * s[4] - c.w_II * s[5];
+        const double aI = c.a_I * s[1] - c.b_I;
+        s[3] = aI / (1 - exp(-c.d_I * aI));
+        const double dSI = n1 * rp[3] * c.sqrt_dt + c.dt_gamma_I * s[3] - c.dt_itau_I * s[5];
+        s[5] += dSI;
+        s[5] = fmax(0.0, fmin(1.0, s[5]));
+    }
+    CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
+                              double n1, const double* t) {
+        (void)n1;
+        s[0] = t[0] + g * gp[0] * rp[1] - t[1];
+        const double aE = c.a_E * s[0] - c.b_E;
+        s[2] = aE / (1 - exp(-c.d_E * aE));
+        const double dSE = n0 * rp[3] * c.sqrt_dt + c.dt_gamma_E * ((1 - s[4]) * s[2]) - c.dt_itau_E * s[4];
+        s[4] += dSE;
+        s[4] = fmax(0.0, fmin(1.0, s[4]));
+    }
 };
 
 // Jansen-Rit ----------------------------------------------------------------
@@ -105,6 +134,32 @@ struct JR {
         s[6] = s[1] - s[2];
         s[7] = c.cmin + (c.cmax - c.cmin) / (1.0 + exp(c.r * (c.midpoint - (s[6]))));
     }
+    // `step` split around the coupling input (see RWW::pre): only y4 reads g.
+    static constexpr int n_tmp = 2;
+    CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double,
+                             double* t) {
+        (void)gp;
+        const double sg0 = 2.0 * c.nu_max / (1.0 + exp(c.r * (c.v0 - (s[1] - s[2]))));
+        const double sg1 = 2.0 * c.nu_max / (1.0 + exp(c.r * (c.v0 - (rp[1] * rp[0] * s[0]))));
+        const double sg2 = 2.0 * c.nu_max / (1.0 + exp(c.r * (c.v0 - (rp[3] * rp[0] * s[0]))));
+        const double d0 = s[3], d1 = s[4], d2 = s[5];
+        const double d3 = c.A * c.a * sg0 - 2.0 * c.a * s[3] - c.a2 * s[0];
+        t[0] = c.mu + rp[2] * rp[0] * sg1;
+        t[1] = c.a2 * s[1];
+        const double d5 = c.B * c.b * (rp[4] * rp[0] * sg2) - 2.0 * c.b * s[5] - c.b2 * s[2];
+        s[0] += c.dt * d0;
+        s[1] += c.dt * d1;
+        s[2] += c.dt * d2;
+        s[3] += c.dt * d3 + c.sqrt_dt * n0 * rp[5];
+        s[5] += c.dt * d5;
+        s[6] = s[1] - s[2];
+        s[7] = c.cmin + (c.cmax - c.cmin) / (1.0 + exp(c.r * (c.midpoint - (s[6]))));
+    }
+    CUBNM_HD static void post(const C& c, double* s, const double* gp, const double*, double g, double, double,
+                              const double* t) {
+        const double d4 = c.A * c.a * (t[0] + gp[0] * g) - 2.0 * c.a * s[4] - t[1];
+        s[4] += c.dt * d4;
+    }
 };
 
 // Kuramoto ------------------------------------------------------------------
@@ -121,6 +176,13 @@ struct Kuramoto {
         const double td = rp[1] + gp[0] * g;
         s[0] = s[0] + c.dt * td + c.sqrt_dt * n0 * rp[2];
         s[0] = fmod(s[0], c.twopi);
+    }
+    // the whole step reads the coupling input
+    static constexpr int n_tmp = 1;
+    CUBNM_HD static void pre(const C&, double*, const double*, const double*, double, double, double*) {}
+    CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
+                              double n1, const double*) {
+        step(c, s, gp, rp, g, n0, n1);
     }
 };
 
@@ -146,6 +208,25 @@ struct WC {
         s[1] += c.dt * dI + c.sqrt_dt * n1 * rp[7];
         s[0] = fmax(0.0, fmin(1.0, s[0]));
         s[1] = fmax(0.0, fmin(1.0, s[1]));
+    }
+    // `step` split around the coupling input (see RWW::pre): the inhibitory
+    // population does not read g.
+    static constexpr int n_tmp = 1;
+    CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double n1,
+                             double* t) {
+        (void)gp; (void)n0;
+        t[0] = rp[0] * s[0] - rp[1] * s[1];
+        const double SI = 1.0 / (1.0 + exp(-c.a_I * (rp[2] * s[0] - rp[3] * s[1] + rp[5] - c.mu_I)));
+        const double dI = (-s[1] + (1.0 - s[1]) * SI) * c.itau_I;
+        s[1] += c.dt * dI + c.sqrt_dt * n1 * rp[7];
+        s[1] = fmax(0.0, fmin(1.0, s[1]));
+    }
+    CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
+                              double, const double* t) {
+        const double SE = 1.0 / (1.0 + exp(-c.a_E * (t[0] + gp[0] * g + rp[4] - c.mu_E)));
+        const double dE = (-s[0] + (1.0 - s[0]) * SE) * c.itau_E;
+        s[0] += c.dt * dE + c.sqrt_dt * n0 * rp[6];
+        s[0] = fmax(0.0, fmin(1.0, s[0]));
     }
 };
 
