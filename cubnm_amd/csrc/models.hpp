@@ -14,6 +14,13 @@
 #include <cmath>
 
 #define CUBNM_HD __host__ __device__ inline
+// Division in the split steps (pre/post) used by the batched integrator:
+// the reciprocal-based sequence on the device, IEEE division on the host.
+#ifdef __HIP_DEVICE_COMPILE__
+#define DIV(a, b) fast_div((a), (b))
+#else
+#define DIV(a, b) ((a) / (b))
+#endif
 
 namespace cubnm {
 
@@ -24,7 +31,18 @@ struct BWConsts {
     double dt, rho, alpha, tau, y, kappa, V_0, k1, k2, k3;
     double ialpha, itau, oneminrho, dt_itau;
     double log_oneminrho;  // log(1 - rho), for the exp/log form of the BW step
+    double irho;           // 1 / rho
 };
+
+// a / b from the hardware reciprocal: one Newton step and one residual
+// correction (6 instructions instead of the 11 of the IEEE division
+// sequence); within one ulp of a / b for normal operands.
+__device__ inline double fast_div(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
 
 // rWW ---------------------------------------------------------------------
 struct RWW {
@@ -85,7 +103,7 @@ struct RWW {
         t[1] = rp[2] * s[5];
         s[1] = c.w_I__I_0 + rp[1] * s[4] - c.w_II * s[5];
         const double aI = c.a_I * s[1] - c.b_I;
-        s[3] = aI / (1 - exp(-c.d_I * aI));
+        s[3] = DIV(aI, 1 - exp(-c.d_I * aI));
         const double dSI = n1 * rp[3] * c.sqrt_dt + c.dt_gamma_I * s[3] - c.dt_itau_I * s[5];
         s[5] += dSI;
         s[5] = fmax(0.0, fmin(1.0, s[5]));
@@ -95,7 +113,7 @@ struct RWW {
         (void)n1;
         s[0] = t[0] + g * gp[0] * rp[1] - t[1];
         const double aE = c.a_E * s[0] - c.b_E;
-        s[2] = aE / (1 - exp(-c.d_E * aE));
+        s[2] = DIV(aE, 1 - exp(-c.d_E * aE));
         const double dSE = n0 * rp[3] * c.sqrt_dt + c.dt_gamma_E * ((1 - s[4]) * s[2]) - c.dt_itau_E * s[4];
         s[4] += dSE;
         s[4] = fmax(0.0, fmin(1.0, s[4]));
@@ -139,9 +157,9 @@ struct JR {
     CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double,
                              double* t) {
         (void)gp;
-        const double sg0 = 2.0 * c.nu_max / (1.0 + exp(c.r * (c.v0 - (s[1] - s[2]))));
-        const double sg1 = 2.0 * c.nu_max / (1.0 + exp(c.r * (c.v0 - (rp[1] * rp[0] * s[0]))));
-        const double sg2 = 2.0 * c.nu_max / (1.0 + exp(c.r * (c.v0 - (rp[3] * rp[0] * s[0]))));
+        const double sg0 = DIV(2.0 * c.nu_max, 1.0 + exp(c.r * (c.v0 - (s[1] - s[2]))));
+        const double sg1 = DIV(2.0 * c.nu_max, 1.0 + exp(c.r * (c.v0 - (rp[1] * rp[0] * s[0]))));
+        const double sg2 = DIV(2.0 * c.nu_max, 1.0 + exp(c.r * (c.v0 - (rp[3] * rp[0] * s[0]))));
         const double d0 = s[3], d1 = s[4], d2 = s[5];
         const double d3 = c.A * c.a * sg0 - 2.0 * c.a * s[3] - c.a2 * s[0];
         t[0] = c.mu + rp[2] * rp[0] * sg1;
@@ -153,7 +171,7 @@ struct JR {
         s[3] += c.dt * d3 + c.sqrt_dt * n0 * rp[5];
         s[5] += c.dt * d5;
         s[6] = s[1] - s[2];
-        s[7] = c.cmin + (c.cmax - c.cmin) / (1.0 + exp(c.r * (c.midpoint - (s[6]))));
+        s[7] = c.cmin + DIV(c.cmax - c.cmin, 1.0 + exp(c.r * (c.midpoint - (s[6]))));
     }
     CUBNM_HD static void post(const C& c, double* s, const double* gp, const double*, double g, double, double,
                               const double* t) {
@@ -216,14 +234,14 @@ struct WC {
                              double* t) {
         (void)gp; (void)n0;
         t[0] = rp[0] * s[0] - rp[1] * s[1];
-        const double SI = 1.0 / (1.0 + exp(-c.a_I * (rp[2] * s[0] - rp[3] * s[1] + rp[5] - c.mu_I)));
+        const double SI = DIV(1.0, 1.0 + exp(-c.a_I * (rp[2] * s[0] - rp[3] * s[1] + rp[5] - c.mu_I)));
         const double dI = (-s[1] + (1.0 - s[1]) * SI) * c.itau_I;
         s[1] += c.dt * dI + c.sqrt_dt * n1 * rp[7];
         s[1] = fmax(0.0, fmin(1.0, s[1]));
     }
     CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
                               double, const double* t) {
-        const double SE = 1.0 / (1.0 + exp(-c.a_E * (t[0] + gp[0] * g + rp[4] - c.mu_E)));
+        const double SE = DIV(1.0, 1.0 + exp(-c.a_E * (t[0] + gp[0] * g + rp[4] - c.mu_E)));
         const double dE = (-s[0] + (1.0 - s[0]) * SE) * c.itau_E;
         s[0] += c.dt * dE + c.sqrt_dt * n0 * rp[6];
         s[0] = fmax(0.0, fmin(1.0, s[0]));
@@ -245,6 +263,28 @@ CUBNM_HD void bw_step(const BWConsts& b, double s, double& x, double& f, double&
 // This matches pow to a few ulp (the BW system is contracting, so the
 // difference stays at rounding level) and costs a third of the device pow.
 // Special values follow pow: log(0) = -inf -> 0, log(x<0) = NaN -> NaN.
+// nu^(1/alpha) for the friston2003 / heinzle alpha = 0.32 (1/alpha = 3.125
+// exactly in FP64): nu^3 * nu^(1/8) with three square roots, a few ulp from
+// pow; any other alpha takes exp(ialpha * log(nu)).
+__device__ inline double bw_nu_pow(const BWConsts& b, double nu) {
+    if (b.ialpha == 3.125) {
+        const double r8 = sqrt(sqrt(sqrt(nu)));
+        return (nu * nu) * nu * r8;
+    }
+    return exp(b.ialpha * log(nu));
+}
+
+// The BW step of bw_step_explog with fewer instructions: powers of nu as
+// above, the divisions by f and nu through fast_div, 1/rho as a product.
+__device__ inline void bw_step_fast(const BWConsts& b, double s, double& x, double& f, double& nu, double& q) {
+    x = x + b.dt * (s - b.kappa * x - b.y * (f - 1.0));
+    const double tmp_f = f + b.dt * x;
+    nu = nu + b.dt_itau * (f - bw_nu_pow(b, nu));
+    const double nu_pow = bw_nu_pow(b, nu);
+    q = q + b.dt_itau * (f * (1.0 - exp(fast_div(b.log_oneminrho, f))) * b.irho - fast_div(nu_pow * q, nu));
+    f = tmp_f;
+}
+
 __device__ inline void bw_step_explog(const BWConsts& b, double s, double& x, double& f, double& nu, double& q) {
     x = x + b.dt * (s - b.kappa * x - b.y * (f - 1.0));
     const double tmp_f = f + b.dt * x;

@@ -527,6 +527,7 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
     a.bw = g_bw;
     a.bw.dt = s.bw_dt; a.bw.dt_itau = a.bw.dt * a.bw.itau;  // core.cpp:291-292
     a.bw.log_oneminrho = std::log(a.bw.oneminrho);
+    a.bw.irho = 1.0 / a.bw.rho;
     bool any_delay = false;
     if (s.do_delay) {
         int mmax = 0;
