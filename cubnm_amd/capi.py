@@ -52,6 +52,11 @@ def lib():
     L.cubnm_gof_fcd_ks.argtypes = [P, I, I, P, I, P]
     L.cubnm_gof_fc_corr_device.argtypes = [P, I, I, P, P]
     L.cubnm_gof_fcd_ks_device.argtypes = [P, I, I, P, I, P]
+    LL = ctypes.POINTER(ctypes.c_longlong)
+    L.cubnm_copy_noise.argtypes = [P, P, P]
+    L.cubnm_rww_grad_sizes.argtypes = [I, I, I, I, D, D, I, LL, LL, LL]
+    L.cubnm_rww_grad_forward.argtypes = [I, I, I, I, D, D, I, I] + [P] * 11
+    L.cubnm_rww_grad_backward.argtypes = [I, I, I, I, D, D, I, I] + [P] * 15
     L.cubnm_init()
     _LIB = L
     return L
@@ -103,3 +108,29 @@ def gof_fc_corr_device(sim_fc, n_sims, n_pairs, emp_fc, out):
 
 def gof_fcd_ks_device(sim_fcd, n_sims, n, emp_sorted, m, out):
     _check(lib().cubnm_gof_fcd_ks_device(sim_fcd, n_sims, n, emp_sorted, m, out))
+
+
+def copy_noise(noise, shuffled_nodes, shuffled_ts):
+    """Host copies (numpy arrays of the sizes in the prepared Dims) of the
+    noise segment and shuffles of the last prepare()."""
+    _check(lib().cubnm_copy_noise(noise.ctypes.data, shuffled_nodes.ctypes.data, shuffled_ts.ctypes.data))
+
+
+def rww_grad_sizes(n_sims, nodes, time_steps, BOLD_TR, dt, bw_dt, ck):
+    b, c, s = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+    _check(lib().cubnm_rww_grad_sizes(n_sims, nodes, time_steps, BOLD_TR, float(dt), float(bw_dt), ck,
+                                      ctypes.byref(b), ctypes.byref(c), ctypes.byref(s)))
+    return b.value, c.value, s.value
+
+
+def rww_grad_forward(n_sims, nodes, time_steps, BOLD_TR, dt, bw_dt, ck, noise_time_steps, *ptrs):
+    """ptrs: SC, G, w_p, J_N, wIE, sigma, noise, shuffled_nodes, shuffled_ts, bold, ckpt (device addresses)."""
+    _check(lib().cubnm_rww_grad_forward(n_sims, nodes, time_steps, BOLD_TR, float(dt), float(bw_dt), ck,
+                                        noise_time_steps, *ptrs))
+
+
+def rww_grad_backward(n_sims, nodes, time_steps, BOLD_TR, dt, bw_dt, ck, noise_time_steps, *ptrs):
+    """ptrs: SC, G, w_p, J_N, wIE, sigma, noise, shuffled_nodes, shuffled_ts, ckpt, dbold, scratch,
+    grad_G, grad_w_p, grad_J_N (device addresses)."""
+    _check(lib().cubnm_rww_grad_backward(n_sims, nodes, time_steps, BOLD_TR, float(dt), float(bw_dt), ck,
+                                         noise_time_steps, *ptrs))

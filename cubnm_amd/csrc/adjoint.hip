@@ -1,0 +1,323 @@
+// Gradients of a BOLD-based loss with respect to rWW parameters through the
+// simulation (no counterpart in the reference, which fits with CMA-ES and
+// grid search only): a hand-written discrete adjoint of the Euler-Maruyama
+// rWW steps, the coupling, the Balloon-Windkessel steps and the BOLD
+// read-out, with the noise held fixed.
+//
+// Forward (`rww_fwd_kernel`): one workgroup per simulation, one lane per
+// node, the same arithmetic as the feature-complete integrator (coupling
+// sum_k SC[k][j] S_E(k) in node order, RWW::step, bw_step, bold_signal), so
+// its BOLD equals run_simulations' bit for bit.  Every `ck` BW steps it
+// stores (S_E, S_I, x, f, nu, q) of every node: a checkpoint.
+//
+// Backward (`rww_adj_kernel`): segments are processed last to first.  Each
+// segment is recomputed from its checkpoint, keeping S_E, S_I and the
+// coupling input g of every inner step and the BW state of every BW step in
+// a per-simulation scratch; then the adjoint runs backwards through it:
+//   BOLD:  y = V0 (k1 (1-q) + k2 (1-q/nu) + k3 (1-nu))
+//   BW:    x' = x + dt (u - kappa x - gamma (f-1)), f' = f + dt x',
+//          nu' = nu + dt/tau (f - nu^a), q' = q + dt/tau (f (1-(1-rho)^(1/f))/rho - nu'^a q / nu')
+//   rWW:   I_E = w_E I_0 + w_p J S_E + G J g - w_IE S_I, I_I = w_I I_0 + J S_E - w_II S_I,
+//          S_E* = S_E + noise + dt gE (1-S_E) H_E(I_E) - dt/tE S_E, S_E' = clip(S_E*, 0, 1),
+//          S_I* = S_I + noise + dt gI H_I(I_I) - dt/tI S_I,         S_I' = clip(S_I*, 0, 1),
+//          H(x) = u / (1 - exp(-d u)), u = a x - b;
+//   coupling: the adjoint of g_j = sum_k SC[k][j] S_E(k) is
+//          lambda_SE(k) += sum_j SC[k][j] e_j with e_j = lambda(I_E(j)) G J_j,
+//          read from an LDS copy of SC (rows padded by one double).
+// Parameter gradients accumulate dI_E/dG = J g, dI_E/dw_p = J S_E,
+// dI_E/dJ = w_p S_E + G g, dI_I/dJ = S_E.  Clipping passes no gradient where
+// it is active.  Memory: checkpoints 6 N doubles per ck BW steps; scratch
+// (3 N inner_it + 5 N) doubles per BW step of one segment, per simulation.
+#include "engine.hpp"
+#include "../../include/cubnm_amd.h"
+#include <string>
+
+namespace cubnm {
+
+BWConsts current_bw();
+void set_last_error(const std::string& e);
+
+namespace {
+
+struct AdjArgs {
+    int n_sims, N, bw_it, inner_it, TR_iters, bold_len, ck, n_ck, noise_bw_it;
+    const double *SC, *G, *wp, *JN, *wIE, *sigma;   // SC [N][N] shared; G [S]; regional [S][N]
+    const double* noise;                            // [noise_bw_it*inner_it][N][2]
+    const int *sh_nodes, *sh_ts;                    // [repeats][N], [repeats][noise_bw_it]
+    double* bold;                                   // [S][bold_len][N] (forward output)
+    double* ckpt;                                   // [S][n_ck][6][N]
+    const double* dbold;                            // [S][bold_len][N] (backward input)
+    double* scratch;                                // [S][ck][inner_it*3 + 5][N]
+    double *gG, *gwp, *gJN;                         // [S], [S][N], [S][N]
+    int sc_lds;                                     // SC staged in LDS (padded rows)
+};
+
+__device__ inline void noise_at(const AdjArgs& a, int bw_i, int inner_i, int j, double* n0, double* n1) {
+    const int rep = bw_i / a.noise_bw_it;
+    const int sh_t = a.sh_ts[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it];
+    const int sh_j = a.sh_nodes[rep * a.N + j];
+    const long long idx = ((long long)(sh_t * a.inner_it + inner_i) * a.N + sh_j) * 2;
+    *n0 = a.noise[idx];
+    *n1 = a.noise[idx + 1];
+}
+
+// derivative of H(x) = u / (1 - exp(-d u)), u = a x - b, with respect to x
+__device__ inline double dH(double a, double b, double d, double x) {
+    const double u = a * x - b;
+    const double e = exp(-d * u);
+    const double den = 1.0 - e;
+    return a * (den - u * d * e) / (den * den);
+}
+}  // namespace
+
+__global__ void __launch_bounds__(1024) rww_fwd_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
+    extern __shared__ double conn[];  // [N]
+    const int sim = blockIdx.x, j = threadIdx.x, N = a.N;
+    const bool act = j < N;
+    const int jj = act ? j : 0;
+    const double gp[1] = {a.G[sim]};
+    const double rp[4] = {a.wp[(long long)sim * N + jj], a.JN[(long long)sim * N + jj],
+                          a.wIE[(long long)sim * N + jj], a.sigma[(long long)sim * N + jj]};
+    double s[RWW::n_state] = {0, 0, 0, 0, 0, 0};
+    RWW::init(c, s, rp);
+    double bx = 0.0, bf = 1.0, bn = 1.0, bq = 1.0;
+    if (act) conn[j] = s[4];
+    double* bold = a.bold + (long long)sim * a.bold_len * N;
+    int bold_i = 0;
+    for (int bw_i = 0; bw_i < a.bw_it; bw_i++) {
+        if (bw_i % a.ck == 0 && act) {
+            double* ck = a.ckpt + ((long long)sim * a.n_ck + bw_i / a.ck) * 6 * N;
+            ck[0 * N + j] = s[4]; ck[1 * N + j] = s[5];
+            ck[2 * N + j] = bx; ck[3 * N + j] = bf; ck[4 * N + j] = bn; ck[5 * N + j] = bq;
+        }
+        for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
+            __syncthreads();
+            double g = 0.0;
+            for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
+            double n0, n1;
+            noise_at(a, bw_i, inner_i, jj, &n0, &n1);
+            RWW::step(c, s, gp, rp, g, n0, n1);
+            __syncthreads();
+            if (act) conn[j] = s[4];
+        }
+        bw_step(bw, s[4], bx, bf, bn, bq);
+        if ((bw_i + 1) % a.TR_iters == 0) {
+            if (act && bold_i < a.bold_len) bold[(long long)bold_i * N + j] = bold_signal(bw, bn, bq);
+            bold_i++;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024) rww_adj_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
+    extern __shared__ double lds[];
+    const int sim = blockIdx.x, j = threadIdx.x, N = a.N;
+    const bool act = j < N;
+    const int jj = act ? j : 0;
+    const int ldsc = N + 1;
+    double* conn = lds;                 // [N]
+    double* ev = lds + N;               // [N]   e_j of the coupling adjoint
+    double* red = lds + 2 * N;          // [16]  block reduction
+    double* scs = lds + 2 * N + 16;     // [N][N+1] SC copy when a.sc_lds
+    if (a.sc_lds)
+        for (int i = threadIdx.x; i < N * N; i += blockDim.x) scs[(i / N) * ldsc + i % N] = a.SC[i];
+    const double G = a.G[sim];
+    const double wp = a.wp[(long long)sim * N + jj], JN = a.JN[(long long)sim * N + jj];
+    const double wIE = a.wIE[(long long)sim * N + jj], sig = a.sigma[(long long)sim * N + jj];
+    const double gp[1] = {G};
+    const double rp[4] = {wp, JN, wIE, sig};
+    const int per_bw = a.inner_it * 3 + 5;
+    double* scr = a.scratch + (long long)sim * a.ck * per_bw * N;
+    const double* dbold = a.dbold + (long long)sim * a.bold_len * N;
+    const double a_dt = bw.dt, a_it = bw.dt_itau, ra = bw.ialpha, r1 = bw.oneminrho;
+    const double lnr1 = log(r1);
+    // adjoint state and gradient accumulators
+    double lSE = 0, lSI = 0, lx = 0, lf = 0, ln = 0, lq = 0;
+    double gGp = 0, gwp = 0, gJN = 0;
+    for (int seg = a.n_ck - 1; seg >= 0; seg--) {
+        const int b0 = seg * a.ck, b1 = min(a.bw_it, b0 + a.ck);
+        // ---- recompute the segment from its checkpoint
+        const double* ck = a.ckpt + ((long long)sim * a.n_ck + seg) * 6 * N;
+        double s[RWW::n_state] = {0, 0, 0, 0, ck[0 * N + jj], ck[1 * N + jj]};
+        double bx = ck[2 * N + jj], bf = ck[3 * N + jj], bn = ck[4 * N + jj], bq = ck[5 * N + jj];
+        __syncthreads();
+        if (act) conn[j] = s[4];
+        for (int b = b0; b < b1; b++) {
+            double* sb = scr + (long long)(b - b0) * per_bw * N;
+            for (int i = 0; i < a.inner_it; i++) {
+                __syncthreads();
+                double g = 0.0;
+                for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
+                if (act) { sb[(3 * i) * N + j] = s[4]; sb[(3 * i + 1) * N + j] = s[5]; sb[(3 * i + 2) * N + j] = g; }
+                double n0, n1;
+                noise_at(a, b, i, jj, &n0, &n1);
+                RWW::step(c, s, gp, rp, g, n0, n1);
+                __syncthreads();
+                if (act) conn[j] = s[4];
+            }
+            double* bwst = sb + (long long)3 * a.inner_it * N;
+            if (act) { bwst[0 * N + j] = bx; bwst[1 * N + j] = bf; bwst[2 * N + j] = bn; bwst[3 * N + j] = bq; bwst[4 * N + j] = s[4]; }
+            bw_step(bw, s[4], bx, bf, bn, bq);
+        }
+        // ---- backwards through the segment
+        for (int b = b1 - 1; b >= b0; b--) {
+            const double* sb = scr + (long long)(b - b0) * per_bw * N;
+            const double* bwst = sb + (long long)3 * a.inner_it * N;
+            const double x = bwst[0 * N + jj], f = bwst[1 * N + jj], nu = bwst[2 * N + jj], q = bwst[3 * N + jj];
+            const double u = bwst[4 * N + jj];
+            // post-BW state
+            double x1 = x, f1 = f, nu1 = nu, q1 = q;
+            bw_step(bw, u, x1, f1, nu1, q1);
+            if ((b + 1) % a.TR_iters == 0) {
+                const int vol = (b + 1) / a.TR_iters - 1;
+                if (vol < a.bold_len) {
+                    const double dy = dbold[(long long)vol * N + jj];
+                    lq += dy * bw.V_0 * (-bw.k1 - bw.k2 / nu1);
+                    ln += dy * bw.V_0 * (bw.k2 * q1 / (nu1 * nu1) - bw.k3);
+                }
+            }
+            // BW step adjoint (lx, lf, ln, lq refer to the post-step state here)
+            const double nu1a = pow(nu1, ra), nua = pow(nu, ra), r1f = pow(r1, 1.0 / f);
+            const double lq_pre = lq * (1.0 - a_it * nu1a / nu1);
+            const double ln1 = ln + lq * (-a_it * q * (ra - 1.0) * nu1a / (nu1 * nu1));
+            double lf_pre = lq * a_it / bw.rho * (1.0 - r1f + r1f * lnr1 / f);
+            const double ln_pre = ln1 * (1.0 - a_it * ra * nua / nu);
+            lf_pre += ln1 * a_it;
+            const double lx1 = lx + lf * a_dt;
+            lf_pre += lf - lx1 * a_dt * bw.y;
+            const double lx_pre = lx1 * (1.0 - a_dt * bw.kappa);
+            const double lu = lx1 * a_dt;
+            lx = lx_pre; lf = lf_pre; ln = ln_pre; lq = lq_pre;
+            lSE += lu;
+            // inner steps, last to first
+            for (int i = a.inner_it - 1; i >= 0; i--) {
+                const double SE = sb[(3 * i) * N + jj], SI = sb[(3 * i + 1) * N + jj], g = sb[(3 * i + 2) * N + jj];
+                double n0, n1;
+                noise_at(a, b, i, jj, &n0, &n1);
+                const double IE = c.w_E__I_0 + wp * JN * SE + g * G * JN - wIE * SI;
+                const double II = c.w_I__I_0 + JN * SE - c.w_II * SI;
+                const double aE = c.a_E * IE - c.b_E, aI = c.a_I * II - c.b_I;
+                const double rE = aE / (1 - exp(-c.d_E * aE)), rI = aI / (1 - exp(-c.d_I * aI));
+                const double SEs = SE + n0 * sig * c.sqrt_dt + c.dt_gamma_E * ((1 - SE) * rE) - c.dt_itau_E * SE;
+                const double SIs = SI + n1 * sig * c.sqrt_dt + c.dt_gamma_I * rI - c.dt_itau_I * SI;
+                const double muE = (SEs > 0.0 && SEs < 1.0) ? lSE : 0.0;
+                const double muI = (SIs > 0.0 && SIs < 1.0) ? lSI : 0.0;
+                const double eE = muE * c.dt_gamma_E * (1 - SE) * dH(c.a_E, c.b_E, c.d_E, IE);
+                const double eI = muI * c.dt_gamma_I * dH(c.a_I, c.b_I, c.d_I, II);
+                gGp += eE * JN * g;
+                gwp += eE * JN * SE;
+                gJN += eE * (wp * SE + G * g) + eI * SE;
+                __syncthreads();
+                if (act) ev[j] = eE * G * JN;
+                __syncthreads();
+                double coup = 0.0;
+                if (a.sc_lds) for (int m = 0; m < N; m++) coup += scs[jj * ldsc + m] * ev[m];
+                else for (int m = 0; m < N; m++) coup += a.SC[(long long)jj * N + m] * ev[m];
+                lSE = muE * (1.0 - c.dt_gamma_E * rE - c.dt_itau_E) + eE * wp * JN + eI * JN + coup;
+                lSI = muI * (1.0 - c.dt_itau_I) - eE * wIE - eI * c.w_II;
+            }
+        }
+    }
+    if (act) {
+        a.gwp[(long long)sim * N + j] = gwp;
+        a.gJN[(long long)sim * N + j] = gJN;
+    }
+    // block sum of the G gradient
+    double v = act ? gGp : 0.0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+        a.gG[sim] = t;
+    }
+}
+
+}  // namespace cubnm
+
+using namespace cubnm;
+
+namespace {
+
+int fill_args(AdjArgs& A, RWW::C& c, BWConsts& bw, int n_sims, int nodes, int time_steps, int BOLD_TR, double dt,
+              double bw_dt, int ck, int noise_time_steps, const double* SC, const double* G, const double* wp,
+              const double* JN, const double* wIE, const double* sigma, const double* noise, const int* sh_nodes,
+              const int* sh_ts) {
+    if (n_sims <= 0 || nodes <= 0 || nodes > 1024) { set_last_error("need 0 < nodes <= 1024 and n_sims > 0"); return -1; }
+    A = AdjArgs{};
+    A.n_sims = n_sims; A.N = nodes;
+    const double bw_dt_s = bw_dt / 1000.0;
+    A.bw_it = (int)(((double)time_steps / 1000.0) / bw_dt_s);
+    A.inner_it = (int)((bw_dt_s * 1000) / dt);
+    A.TR_iters = (int)(((double)BOLD_TR / 1000.0) / bw_dt_s);
+    A.bold_len = time_steps / BOLD_TR;
+    A.noise_bw_it = (int)(((double)noise_time_steps / 1000.0) / bw_dt_s);
+    if (A.bw_it <= 0 || A.inner_it <= 0 || A.TR_iters <= 0 || A.noise_bw_it <= 0 || ck <= 0) {
+        set_last_error("invalid timing or checkpoint interval"); return -1;
+    }
+    A.ck = ck; A.n_ck = (A.bw_it + ck - 1) / ck;
+    A.SC = SC; A.G = G; A.wp = wp; A.JN = JN; A.wIE = wIE; A.sigma = sigma;
+    A.noise = noise; A.sh_nodes = sh_nodes; A.sh_ts = sh_ts;
+    RWW::init_constants(c, dt);
+    bw = current_bw();
+    bw.dt = bw_dt_s; bw.dt_itau = bw.dt * bw.itau;
+    return 0;
+}
+
+int finish(hipError_t e) {
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) { set_last_error(std::string("HIP error: ") + hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cubnm_rww_grad_sizes(int n_sims, int nodes, int time_steps, int BOLD_TR, double dt, double bw_dt, int ck,
+                         long long* bold_doubles, long long* ckpt_doubles, long long* scratch_doubles) {
+    AdjArgs A; RWW::C c; BWConsts bw;
+    if (fill_args(A, c, bw, n_sims, nodes, time_steps, BOLD_TR, dt, bw_dt, ck, time_steps, nullptr, nullptr, nullptr,
+                  nullptr, nullptr, nullptr, nullptr, nullptr, nullptr) != 0)
+        return -1;
+    *bold_doubles = (long long)n_sims * A.bold_len * nodes;
+    *ckpt_doubles = (long long)n_sims * A.n_ck * 6 * nodes;
+    *scratch_doubles = (long long)n_sims * ck * (A.inner_it * 3 + 5) * nodes;
+    return 0;
+}
+
+int cubnm_rww_grad_forward(int n_sims, int nodes, int time_steps, int BOLD_TR, double dt, double bw_dt, int ck,
+                           int noise_time_steps, const double* d_SC, const double* d_G, const double* d_wp,
+                           const double* d_JN, const double* d_wIE, const double* d_sigma, const double* d_noise,
+                           const int* d_sh_nodes, const int* d_sh_ts, double* d_bold, double* d_ckpt) {
+    AdjArgs A; RWW::C c; BWConsts bw;
+    if (fill_args(A, c, bw, n_sims, nodes, time_steps, BOLD_TR, dt, bw_dt, ck, noise_time_steps, d_SC, d_G, d_wp, d_JN,
+                  d_wIE, d_sigma, d_noise, d_sh_nodes, d_sh_ts) != 0)
+        return -1;
+    A.bold = d_bold; A.ckpt = d_ckpt;
+    const int threads = ((nodes + 63) / 64) * 64;
+    hipLaunchKernelGGL(rww_fwd_kernel, dim3(n_sims), dim3(threads), sizeof(double) * nodes, 0, A, c, bw);
+    return finish(hipGetLastError());
+}
+
+int cubnm_rww_grad_backward(int n_sims, int nodes, int time_steps, int BOLD_TR, double dt, double bw_dt, int ck,
+                            int noise_time_steps, const double* d_SC, const double* d_G, const double* d_wp,
+                            const double* d_JN, const double* d_wIE, const double* d_sigma, const double* d_noise,
+                            const int* d_sh_nodes, const int* d_sh_ts, const double* d_ckpt, const double* d_dbold,
+                            double* d_scratch, double* d_gG, double* d_gwp, double* d_gJN) {
+    AdjArgs A; RWW::C c; BWConsts bw;
+    if (fill_args(A, c, bw, n_sims, nodes, time_steps, BOLD_TR, dt, bw_dt, ck, noise_time_steps, d_SC, d_G, d_wp, d_JN,
+                  d_wIE, d_sigma, d_noise, d_sh_nodes, d_sh_ts) != 0)
+        return -1;
+    A.ckpt = const_cast<double*>(d_ckpt); A.dbold = d_dbold; A.scratch = d_scratch;
+    A.gG = d_gG; A.gwp = d_gwp; A.gJN = d_gJN;
+    const size_t base = sizeof(double) * (2 * (size_t)nodes + 16);
+    const size_t with_sc = base + sizeof(double) * (size_t)nodes * (nodes + 1);
+    A.sc_lds = with_sc <= 150 * 1024;
+    const int threads = ((nodes + 63) / 64) * 64;
+    hipLaunchKernelGGL(rww_adj_kernel, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
+    return finish(hipGetLastError());
+}
+
+}  // extern "C"

@@ -739,6 +739,16 @@ double group_cost(const cubnm_group& g) {
 
 void set_last_error(const std::string& e) { g_err = e; }
 
+// Balloon-Windkessel constants as set by cubnm_init / cubnm_set_const, with
+// the derived fields of the device steps (the caller sets dt and dt_itau).
+BWConsts current_bw() {
+    if (!g_bw_ready) { bw_defaults(g_bw); g_bw_ready = true; }
+    BWConsts b = g_bw;
+    b.log_oneminrho = std::log(b.oneminrho);
+    b.irho = 1.0 / b.rho;
+    return b;
+}
+
 }  // namespace cubnm
 
 using namespace cubnm;
@@ -778,6 +788,20 @@ int cubnm_set_tuning(const char* key, int value) {
     g_err = std::string("unknown tuning key ") + key;
     return -1;
 }
+int cubnm_copy_noise(double* noise, int* shuffled_nodes, int* shuffled_ts) {
+    try {
+        require_session();
+        const Session& s = g_s;
+        if (noise) std::copy(s.noise_e->noise.begin(), s.noise_e->noise.end(), noise);
+        if (shuffled_nodes) std::copy(s.sh_nodes.begin(), s.sh_nodes.end(), shuffled_nodes);
+        if (shuffled_ts) std::copy(s.sh_ts.begin(), s.sh_ts.end(), shuffled_ts);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 int cubnm_last_integrator(void) { return g_s.last_integrator; }
 int cubnm_last_fc_path(void) { return g_s.last_fc_path; }
 

@@ -151,6 +151,34 @@ int cubnm_last_fc_path(void);
 
 const char* cubnm_last_error(void);
 
+/* Host copies of the noise segment and shuffles of the last cubnm_prepare
+ * (the _noise, _shuffled_nodes and _shuffled_ts outputs of run_simulations
+ * without running it).  Sizes: dims.noise_size, noise_repeats*nodes,
+ * noise_repeats*noise_bw_it. */
+int cubnm_copy_noise(double* noise, int* shuffled_nodes, int* shuffled_ts);
+
+/* Gradients through the rWW simulation (no counterpart in the reference):
+ * a discrete adjoint of the Euler-Maruyama rWW steps, coupling,
+ * Balloon-Windkessel steps and BOLD read-out with the noise held fixed
+ * (noise and shuffles as from cubnm_copy_noise, noise width 2).  Device
+ * pointers; one SC [nodes][nodes] shared by all simulations; G [n_sims];
+ * w_p, J_N, wIE, sigma [n_sims][nodes]; BOLD and dL/dBOLD
+ * [n_sims][bold_len][nodes].  Timing arguments as cubnm_prepare.  The
+ * forward pass stores a checkpoint every `ck` BW steps; the backward pass
+ * recomputes one segment at a time.  Buffer sizes (doubles) come from
+ * cubnm_rww_grad_sizes.  No FIC, no delay (wIE is an input). */
+int cubnm_rww_grad_sizes(int n_sims, int nodes, int time_steps, int BOLD_TR, double dt, double bw_dt, int ck,
+                         long long* bold_doubles, long long* ckpt_doubles, long long* scratch_doubles);
+int cubnm_rww_grad_forward(int n_sims, int nodes, int time_steps, int BOLD_TR, double dt, double bw_dt, int ck,
+                           int noise_time_steps, const double* d_SC, const double* d_G, const double* d_wp,
+                           const double* d_JN, const double* d_wIE, const double* d_sigma, const double* d_noise,
+                           const int* d_sh_nodes, const int* d_sh_ts, double* d_bold, double* d_ckpt);
+int cubnm_rww_grad_backward(int n_sims, int nodes, int time_steps, int BOLD_TR, double dt, double bw_dt, int ck,
+                            int noise_time_steps, const double* d_SC, const double* d_G, const double* d_wp,
+                            const double* d_JN, const double* d_wIE, const double* d_sigma, const double* d_noise,
+                            const int* d_sh_nodes, const int* d_sh_ts, const double* d_ckpt, const double* d_dbold,
+                            double* d_scratch, double* d_gG, double* d_gwp, double* d_gJN);
+
 /* Goodness of fit on the GPU (utils.py:306-507 fc_corr_device and
  * fcd_ks_device, sim/base.py:1084-1207): Pearson r between each simulated FC
  * tril and the empirical one, and the two-sample Kolmogorov-Smirnov
