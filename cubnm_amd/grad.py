@@ -47,7 +47,7 @@ class RwwProblem:
         sht = np.empty(d.noise_repeats * d.noise_bw_it, np.intc)
         capi.copy_noise(noise, shn, sht)
         self.bold_len = d.bold_len
-        t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+        t = lambda a, dt=torch.float64: torch.as_tensor(np.array(a, copy=True), dtype=dt, device=device)
         self.sc, self.wIE, self.sigma = t(sc), t(wIE), t(np.broadcast_to(np.asarray(sigma, float), wIE.shape))
         self.noise, self.shn, self.sht = t(noise), t(shn, torch.int32), t(sht, torch.int32)
         nb, nc, ns = capi.rww_grad_sizes(self.n_sims, self.nodes, self.time_steps, self.TR, self.dt, self.bw_dt, self.ck)

@@ -1,7 +1,9 @@
 """Gradients through the rWW simulation (adjoint kernel) on the GPU.
 
-* The differentiable forward pass reproduces run_simulations' BOLD exactly
-  (feature-complete integrator, same noise, FIC off, wIE given).
+* The differentiable forward pass reproduces run_simulations' BOLD
+  (feature-complete integrator, same noise, FIC off, wIE given) to 1e-12
+  relative: the same operations, contracted into FMAs differently by the
+  compiler in the two kernels (observed 1.5e-14).
 * The adjoint gradient of L = sum(W * BOLD) with respect to G, w_p and J_N
   matches central finite differences of the same forward pass
   (h = 1e-6 relative), with checkpoint segments shorter than the run so the
@@ -54,7 +56,7 @@ def test_forward_matches_run_simulations():
         out = _core.run_simulations(*args)
     finally:
         _core.force_integrator(-1)
-    np.testing.assert_array_equal(bold.reshape(S, -1), out["sim_bold"])
+    assert cases.bold_rel_err(bold.reshape(S, -1), out["sim_bold"]).max() <= 1e-12
 
 
 def test_adjoint_matches_finite_differences():
