@@ -109,6 +109,11 @@ def main():
     ap.add_argument("--tpw", type=int, default=0, help="batched integrator tiles per wave (0: auto)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (PMC byte-counter calibration)")
+    ap.add_argument("--workload", choices=["c2", "c4"], default="c2",
+                    help="c2: the grid throughput (default); c4: gradient fit vs CMA-ES time-to-loss")
+    ap.add_argument("--c4-gens", type=int, default=60, help="CMA-ES generations (c4)")
+    ap.add_argument("--c4-steps", type=int, default=40, help="maximum Adam steps (c4)")
+    ap.add_argument("--c4-duration", type=float, default=120.0, help="simulated seconds (c4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X and the simulations shard "
@@ -118,6 +123,9 @@ def main():
     import torch
     import cases
     from cubnm_amd import capi, utils
+
+    if args.workload == "c4":
+        return run_c4(args)
 
     dev = torch.device("cuda:0")
     S, N, dur = args.sims, args.nodes, args.duration
@@ -210,6 +218,52 @@ def main():
                    "sims": S, "nodes": N, "duration_s": dur, "integrator": "batched MFMA FP64"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_c4(args):
+    """Regional fit of rWW (G + w_p[100] + J_N[100], unit-cube bounds G in
+    [0.001, 10], w_p in [0, 2], J_N in [0.001, 0.5]; FIC off, wIE = 1) to a
+    seeded AR(1) empirical BOLD of 100 x 166 volumes at TR 0.72 s, 120 s
+    simulations at dt 0.1 ms, loss (1 - FC corr) + mean |FCD quantile
+    differences| (64 quantiles; FCD windows 42 volumes, step 7, 41 volumes of
+    burn-in).  CMA-ES (popsize 64, sigma 0.5) runs for --c4-gens generations;
+    Adam on 256 starts (one adjoint pass per step, checkpoint every 1000 BW
+    steps) runs until it reaches CMA-ES's best loss.  value = CMA-ES wall time
+    to its best loss / gradient wall time to the same loss (higher is better)."""
+    import torch
+    import cases
+    from cubnm_amd import fit, grad
+    N, TR_s, dur = 100, 0.72, args.c4_duration
+    sc = cases.synthetic_sc(N, seed=0)
+    emp = empirical_bold(N, int(dur * 1000) // int(TR_s * 1000), seed=4)
+    bounds = {"G": (0.001, 10.0), "w_p": (0.0, 2.0), "J_N": (0.001, 0.5)}
+    nvr = int(30 * 1000 / (TR_s * 1000))
+    mk = lambda n: grad.RwwProblem(sc, np.ones((n, N)), 0.01, dur, TR_s, seed=0, checkpoint_bw_steps=1000)
+    fit_cma = fit.RegionalFit(mk(64), emp.T, bounds, 42, 7, nvr)
+    log("c4: CMA-ES")
+    h_cma = fit.cmaes_fit(fit_cma, args.c4_gens, seed=1, log=log)
+    best = h_cma[-1][1]
+    t_cma = next(t for t, b in h_cma if b <= best)
+    del fit_cma
+    torch.cuda.empty_cache()
+    fit_gd = fit.RegionalFit(mk(256), emp.T, bounds, 42, 7, nvr)
+    log("c4: Adam")
+    h_gd = fit.adam_fit(fit_gd, args.c4_steps, lr=0.02, seed=2, target=best, log=log)
+    reached = [t for t, b in h_gd if b <= best]
+    t_gd = reached[0] if reached else None
+    line = {
+        "metric": METRIC, "value": round(t_cma / t_gd, 3) if t_gd else 0.0, "unit": "x (CMA-ES time / gradient time to CMA-ES best loss)",
+        "n_gpus": 1, "steps": len(h_gd), "warmup": 0, "ms_per_step": round(1000 * h_gd[-1][0] / len(h_gd), 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded lognormal SC, seeded AR(1) empirical BOLD 100x166 at TR 0.72 s, host-generated noise)",
+        "config": {"workload": f"rWW regional fit, 201 parameters, N={N}, {dur:g} s, TR 0.72 s; CMA-ES popsize 64 "
+                               f"x {args.c4_gens} generations vs Adam on 256 starts with adjoint gradients",
+                   "cmaes_best_loss": round(best, 6), "cmaes_seconds_to_best": round(t_cma, 2),
+                   "grad_seconds_to_cmaes_best": round(t_gd, 2) if t_gd else None,
+                   "grad_best_loss": round(h_gd[-1][1], 6), "grad_steps": len(h_gd)},
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 
