@@ -241,14 +241,14 @@ def run_c4(args):
     bounds = {"G": (0.001, 10.0), "w_p": (0.0, 2.0), "J_N": (0.001, 0.5)}
     nvr = int(30 * 1000 / (TR_s * 1000))
     mk = lambda n: grad.RwwProblem(sc, np.ones((n, N)), 0.01, dur, TR_s, seed=0, checkpoint_bw_steps=1000)
-    fit_cma = fit.RegionalFit(mk(64), emp.T, bounds, 42, 7, nvr)
+    fit_cma = fit.RegionalFit(mk(64), emp, bounds, 42, 7, nvr)
     log("c4: CMA-ES")
     h_cma = fit.cmaes_fit(fit_cma, args.c4_gens, seed=1, log=log)
     best = h_cma[-1][1]
     t_cma = next(t for t, b in h_cma if b <= best)
     del fit_cma
     torch.cuda.empty_cache()
-    fit_gd = fit.RegionalFit(mk(256), emp.T, bounds, 42, 7, nvr)
+    fit_gd = fit.RegionalFit(mk(256), emp, bounds, 42, 7, nvr)
     log("c4: Adam")
     h_gd = fit.adam_fit(fit_gd, args.c4_steps, lr=0.02, seed=2, target=best, log=log)
     reached = [t for t, b in h_gd if b <= best]

@@ -111,8 +111,9 @@ class CMAES:
 
 class RegionalFit:
     """G (1) + w_p (N) + J_N (N) in the unit cube for one SC and empirical
-    BOLD; ``evaluate(U)`` returns the loss of each row of U (a batch the
-    size of the problem), ``loss_and_grad(U)`` also its gradient."""
+    BOLD (nodes x volumes, as the reference's emp_bold); ``evaluate(U)``
+    returns the loss of each row of U (a batch the size of the problem),
+    ``loss_and_grad(U)`` also its gradient."""
 
     def __init__(self, problem, emp_bold, bounds, window, step, n_vols_remove):
         self.p = problem
@@ -124,7 +125,7 @@ class RegionalFit:
                                                np.full(self.N, bounds["J_N"][1])]), dtype=torch.float64, device=dev)
         self.nvr = n_vols_remove
         self.windows = windows_for(problem.bold_len, window, step, n_vols_remove)
-        eb = torch.tensor(np.asarray(emp_bold, float), dtype=torch.float64, device=dev)[None]
+        eb = torch.tensor(np.asarray(emp_bold, float).T.copy(), dtype=torch.float64, device=dev)[None]
         self.emp_fc = grad.fc_tril(eb, n_vols_remove)[0]
         qs = torch.linspace(0, 1, 64, dtype=torch.float64, device=dev)
         self.emp_q = torch.quantile(fcd_tril(eb, self.windows)[0], qs)
