@@ -71,10 +71,15 @@ __device__ inline double dH(double a, double b, double d, double x) {
 }  // namespace
 
 __global__ void __launch_bounds__(1024) rww_fwd_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
-    extern __shared__ double conn[];  // [N]
+    extern __shared__ double lds[];
     const int sim = blockIdx.x, j = threadIdx.x, N = a.N;
     const bool act = j < N;
     const int jj = act ? j : 0;
+    double* conn = lds;               // [N]
+    double* scs = lds + N;            // [N][N+1] SC copy when a.sc_lds
+    const int ldsc = N + 1;
+    if (a.sc_lds)
+        for (int i = threadIdx.x; i < N * N; i += blockDim.x) scs[(i / N) * ldsc + i % N] = a.SC[i];
     const double gp[1] = {a.G[sim]};
     const double rp[4] = {a.wp[(long long)sim * N + jj], a.JN[(long long)sim * N + jj],
                           a.wIE[(long long)sim * N + jj], a.sigma[(long long)sim * N + jj]};
@@ -93,7 +98,13 @@ __global__ void __launch_bounds__(1024) rww_fwd_kernel(AdjArgs a, RWW::C c, BWCo
         for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
             __syncthreads();
             double g = 0.0;
-            for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
+            if (a.sc_lds) {
+#pragma unroll 8
+                for (int k = 0; k < N; k++) g += scs[k * ldsc + jj] * conn[k];
+            } else {
+#pragma unroll 8
+                for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
+            }
             double n0, n1;
             noise_at(a, bw_i, inner_i, jj, &n0, &n1);
             RWW::step(c, s, gp, rp, g, n0, n1);
@@ -146,7 +157,13 @@ __global__ void __launch_bounds__(1024) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
             for (int i = 0; i < a.inner_it; i++) {
                 __syncthreads();
                 double g = 0.0;
-                for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
+                if (a.sc_lds) {
+#pragma unroll 8
+                    for (int k = 0; k < N; k++) g += scs[k * ldsc + jj] * conn[k];
+                } else {
+#pragma unroll 8
+                    for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
+                }
                 if (act) { sb[(3 * i) * N + j] = s[4]; sb[(3 * i + 1) * N + j] = s[5]; sb[(3 * i + 2) * N + j] = g; }
                 double n0, n1;
                 noise_at(a, b, i, jj, &n0, &n1);
@@ -210,8 +227,13 @@ __global__ void __launch_bounds__(1024) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
                 if (act) ev[j] = eE * G * JN;
                 __syncthreads();
                 double coup = 0.0;
-                if (a.sc_lds) for (int m = 0; m < N; m++) coup += scs[jj * ldsc + m] * ev[m];
-                else for (int m = 0; m < N; m++) coup += a.SC[(long long)jj * N + m] * ev[m];
+                if (a.sc_lds) {
+#pragma unroll 8
+                    for (int m = 0; m < N; m++) coup += scs[jj * ldsc + m] * ev[m];
+                } else {
+#pragma unroll 8
+                    for (int m = 0; m < N; m++) coup += a.SC[(long long)jj * N + m] * ev[m];
+                }
                 lSE = muE * (1.0 - c.dt_gamma_E * rE - c.dt_itau_E) + eE * wp * JN + eI * JN + coup;
                 lSI = muI * (1.0 - c.dt_itau_I) - eE * wIE - eI * c.w_II;
             }
@@ -296,8 +318,11 @@ int cubnm_rww_grad_forward(int n_sims, int nodes, int time_steps, int BOLD_TR, d
                   d_wIE, d_sigma, d_noise, d_sh_nodes, d_sh_ts) != 0)
         return -1;
     A.bold = d_bold; A.ckpt = d_ckpt;
+    const size_t base = sizeof(double) * (size_t)nodes;
+    const size_t with_sc = base + sizeof(double) * (size_t)nodes * (nodes + 1);
+    A.sc_lds = with_sc <= 150 * 1024;
     const int threads = ((nodes + 63) / 64) * 64;
-    hipLaunchKernelGGL(rww_fwd_kernel, dim3(n_sims), dim3(threads), sizeof(double) * nodes, 0, A, c, bw);
+    hipLaunchKernelGGL(rww_fwd_kernel, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
     return finish(hipGetLastError());
 }
 

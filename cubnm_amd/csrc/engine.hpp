@@ -35,6 +35,7 @@ struct IntegrateArgs {
     double* hist;             // circular history, hist_stride doubles per sim
     long long hist_stride;
     BWConsts bw;
+    int sc_lds;               // general kernel: SC staged in LDS (rows padded by one)
 };
 
 struct FcArgs {

@@ -37,9 +37,15 @@ __global__ void __launch_bounds__(1024) integrate_general_kernel(IntegrateArgs a
     const int N = a.nodes;
     const bool act = j < N;
     const int jj = act ? j : 0;  // inactive lanes mirror node 0 but never publish
-    extern __shared__ double conn[];  // [N] previous-step coupling state (no delay)
-
-    const double* SC = a.SC + (long long)a.SC_indices[sim] * N * N;
+    extern __shared__ double conn[];  // [N] previous-step coupling state (no delay), then SC when staged
+    const double* SCg = a.SC + (long long)a.SC_indices[sim] * N * N;
+    // SC in LDS (rows padded by one double) when it fits: the coupling loop
+    // then reads LDS instead of L1/L2 every step; summation order unchanged
+    double* scs = conn + N;
+    const int ldsc = a.sc_lds ? N + 1 : N;
+    if (a.sc_lds)
+        for (int i = threadIdx.x; i < N * N; i += blockDim.x) scs[(i / N) * ldsc + i % N] = SCg[i];
+    const double* SC = a.sc_lds ? scs : SCg;
     double gp[M::n_global], rp[M::n_regional];
     for (int i = 0; i < M::n_global; i++) gp[i] = a.gp[(long long)i * a.N_SIMS + sim];
     for (int i = 0; i < M::n_regional; i++) rp[i] = a.rp[(long long)i * a.N_SIMS * N + (long long)sim * N + jj];
@@ -88,14 +94,15 @@ __global__ void __launch_bounds__(1024) integrate_general_kernel(IntegrateArgs a
                 for (int k = 0; k < N; k++) {
                     const int kb = (buff_idx + delay[(long long)k * N + jj]) % max_delay;
                     const double hk = hist[(long long)kb * N + k];
-                    if constexpr (M::is_osc) g += SC[(long long)k * N + jj] * sin(hk - sj);
-                    else g += SC[(long long)k * N + jj] * hk;
+                    if constexpr (M::is_osc) g += SC[(long long)k * ldsc + jj] * sin(hk - sj);
+                    else g += SC[(long long)k * ldsc + jj] * hk;
                 }
             } else {
                 const double sj = conn[jj];
+#pragma unroll 8
                 for (int k = 0; k < N; k++) {
-                    if constexpr (M::is_osc) g += SC[(long long)k * N + jj] * sin(conn[k] - sj);
-                    else g += SC[(long long)k * N + jj] * conn[k];
+                    if constexpr (M::is_osc) g += SC[(long long)k * ldsc + jj] * sin(conn[k] - sj);
+                    else g += SC[(long long)k * ldsc + jj] * conn[k];
                 }
             }
             const long long nidx = ((long long)(sh_t * a.inner_it + inner_i) * N + sh_j) * M::n_noise;
@@ -193,8 +200,14 @@ template <class M>
 static hipError_t launch_general(const IntegrateArgs& a, const void* consts, hipStream_t st) {
     const typename M::C mc = *static_cast<const typename M::C*>(consts);
     int threads = ((a.nodes + 63) / 64) * 64;
+    IntegrateArgs b = a;
     size_t lds = sizeof(double) * (size_t)a.nodes;
-    hipLaunchKernelGGL(integrate_general_kernel<M>, dim3(a.N_SIMS), dim3(threads), lds, st, a, mc);
+    const size_t with_sc = lds + sizeof(double) * (size_t)a.nodes * (a.nodes + 1);
+    // staged when it leaves room for two workgroups per CU, or when there are
+    // too few simulations to fill the CUs anyway (latency-bound)
+    b.sc_lds = with_sc <= 64 * 1024 || (with_sc <= 150 * 1024 && a.N_SIMS <= 512);
+    if (b.sc_lds) lds = with_sc;
+    hipLaunchKernelGGL(integrate_general_kernel<M>, dim3(a.N_SIMS), dim3(threads), lds, st, b, mc);
     return hipGetLastError();
 }
 
