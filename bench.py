@@ -109,8 +109,10 @@ def main():
     ap.add_argument("--tpw", type=int, default=0, help="batched integrator tiles per wave (0: auto)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (PMC byte-counter calibration)")
-    ap.add_argument("--workload", choices=["c2", "c4"], default="c2",
-                    help="c2: the grid throughput (default); c4: gradient fit vs CMA-ES time-to-loss")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2",
+                    help="c2: the grid throughput (default); c3: heterogeneous batch; "
+                         "c4: gradient fit vs CMA-ES time-to-loss")
+    ap.add_argument("--c3-sims", type=int, default=4096, help="simulations of the heterogeneous batch (c3)")
     ap.add_argument("--c4-gens", type=int, default=60, help="CMA-ES generations (c4)")
     ap.add_argument("--c4-steps", type=int, default=40, help="maximum Adam steps (c4)")
     ap.add_argument("--c4-duration", type=float, default=120.0, help="simulated seconds (c4)")
@@ -126,6 +128,8 @@ def main():
 
     if args.workload == "c4":
         return run_c4(args)
+    if args.workload == "c3":
+        return run_c3(args)
 
     dev = torch.device("cuda:0")
     S, N, dur = args.sims, args.nodes, args.duration
@@ -218,6 +222,57 @@ def main():
                    "sims": S, "nodes": N, "duration_s": dur, "integrator": "batched MFMA FP64"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_c3(args):
+    """Heterogeneous batch through one call of the grouped entry point:
+    --c3-sims simulations (4096 in the full configuration), node count
+    uniform in {68, 100, 200, 400}, duration uniform in {60, 120, 300, 600} s,
+    models 70% rWW / 15% JR / 15% Kuramoto, each simulation with its own
+    seeded lognormal SC and uniform G in [0.001, 5] (other parameters at their
+    defaults, rWW FIC off), TR 0.72 s, FCD windows 30 TRs step 5, 30 s burn-in.
+    Simulations are bucketed by (model, nodes, duration), one request per
+    bucket; value = total sim-node-seconds / wall seconds of the call."""
+    import torch  # noqa: F401
+    import cases
+    from cubnm_amd import _core
+    rng = np.random.default_rng(3)
+    S = args.c3_sims
+    Ns = rng.choice([68, 100, 200, 400], S)
+    durs = rng.choice([60, 120, 300, 600], S)
+    models = rng.choice(["rWW", "JR", "Kuramoto"], S, p=[0.7, 0.15, 0.15])
+    reqs, work = [], 0.0
+    for m in ["rWW", "JR", "Kuramoto"]:
+        for n in [68, 100, 200, 400]:
+            for d in [60, 120, 300, 600]:
+                idx = np.where((models == m) & (Ns == n) & (durs == d))[0]
+                if idx.size == 0:
+                    continue
+                k = idx.size
+                scs = np.stack([cases.synthetic_sc(n, seed=int(1000 + i)) for i in idx])
+                gp, rp = cases.params(m, k, n, G=rng.uniform(0.001, 5.0, k), do_fic=False)
+                cfg = cases.model_config(m, TR=0.72, bold_remove_s=30, do_fic=False)
+                cfg.update({"window_size": "30", "window_step": "5"})
+                reqs.append(cases.sim_args(m, scs, gp, rp, cfg, k, duration_s=d, TR_s=0.72,
+                                           sc_indices=np.arange(k, dtype=np.intc)))
+                work += k * n * d
+    log(f"c3: {S} simulations in {len(reqs)} requests, {work:.3e} sim-node-s")
+    _core.core.run_groups(reqs[:1])  # session and noise set-up for the first bucket (untimed)
+    t0 = time.time()
+    outs = _core.core.run_groups(reqs)
+    wall = time.time() - t0
+    assert len(outs) == len(reqs)
+    line = {
+        "metric": METRIC, "value": round(work / wall, 1), "unit": "sim-node-s/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": round(1000 * wall, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded lognormal SC per simulation, host-generated noise)",
+        "config": {"workload": f"heterogeneous batch: {S} sims, N in {{68,100,200,400}}, 60-600 s, rWW/JR/Kuramoto "
+                               f"70/15/15, own SC each, TR 0.72 s, FCD 30/5 TRs, {len(reqs)} requests in one call",
+                   "sim_node_seconds": work},
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

@@ -544,6 +544,16 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
     a.do_delay = any_delay;
     const bool fic_restarts = s.model->id == MODEL_RWW && s.rww.do_fic && s.rww.max_fic_trials > 0;
     bool use_batched = !any_delay && !fic_restarts && !s.conf.states_ts && batched_supported(s.model->id, N);
+    if (use_batched && g_force_integrator != 1) {
+        // groups of 16 share one SC: with many SCs and few simulations per SC
+        // most lanes would be padding; past 2x padding the per-simulation
+        // integrator does less work
+        std::vector<int> per_sc(s.N_SCs, 0);
+        for (int i = 0; i < s.N_SIMS; i++) per_sc[SCi_host[i]]++;
+        long long slots = 0;
+        for (int c : per_sc) slots += 16LL * ((c + 15) / 16);
+        if (slots > 2LL * s.N_SIMS) use_batched = false;
+    }
     if (g_force_integrator == 0) use_batched = false;
     if (g_force_integrator == 1 && !use_batched)
         throw std::runtime_error("batched integrator does not support this configuration (delay, numerical FIC, states_ts or node count)");
