@@ -254,7 +254,7 @@ def run_c3(args):
                 scs = np.stack([cases.synthetic_sc(n, seed=int(1000 + i)) for i in idx])
                 gp, rp = cases.params(m, k, n, G=rng.uniform(0.001, 5.0, k), do_fic=False)
                 cfg = cases.model_config(m, TR=0.72, bold_remove_s=30, do_fic=False)
-                cfg.update({"window_size": "30", "window_step": "5"})
+                cfg.update({"window_size": "30", "window_step": "5", "verbose": "1"})
                 reqs.append(cases.sim_args(m, scs, gp, rp, cfg, k, duration_s=d, TR_s=0.72,
                                            sc_indices=np.arange(k, dtype=np.intc)))
                 work += k * n * d

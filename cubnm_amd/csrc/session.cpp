@@ -16,6 +16,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -101,6 +102,7 @@ struct BaseConf {
     bool drop_edges = true;
     int noise_time_steps = 30000;
     bool ext_out = true, states_ts = false;
+    bool verbose = false;
 };
 
 // ---------------------------------------------------------------- noise cache
@@ -186,7 +188,8 @@ void apply_config(Session& s, const std::vector<std::pair<std::string, std::stri
         else if (k == "window_step") s.conf.window_step = parse_int(v);
         else if (k == "drop_edges") s.conf.drop_edges = (bool)parse_int(v);
         else if (k == "noise_time_steps") s.conf.noise_time_steps = parse_int(v);
-        // verbose / progress_interval are accepted and ignored (no progress polling)
+        else if (k == "verbose") s.conf.verbose = (bool)parse_int(v);
+        // progress_interval is accepted and ignored (the integrators are not polled)
     }
     s.modifies_params = false;
     if (s.model->id == MODEL_RWW) {
@@ -893,13 +896,33 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
                 throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
             }
         }
-        for (int k = 0; k < n_groups; k++) {
-            const int i = order[k];
-            try {
-                finish_run(*g_pool[i], &groups[i].out);
-            } catch (const std::exception& e) {
-                throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
+        // collect groups as they complete; with "verbose" in any group's
+        // config, report each completion (sim-node-seconds and elapsed time)
+        bool verbose = false;
+        for (int i = 0; i < n_groups; i++) verbose = verbose || g_pool[i]->conf.verbose;
+        std::vector<char> done(n_groups, 0);
+        const double t0 = now_s();
+        for (int left = n_groups; left > 0;) {
+            bool progressed = false;
+            for (int k = 0; k < n_groups; k++) {
+                const int i = order[k];
+                if (done[i]) continue;
+                const hipError_t q = hipStreamQuery(g_pool[i]->stream);
+                if (q == hipErrorNotReady) continue;
+                try {
+                    finish_run(*g_pool[i], &groups[i].out);
+                } catch (const std::exception& e) {
+                    throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
+                }
+                done[i] = 1;
+                left--;
+                progressed = true;
+                if (verbose)
+                    std::fprintf(stderr, "cubnm_run_groups: group %d done (%s, %d sims x %d nodes x %.1f s) at %.1f s, %d left\n",
+                                 i, g_pool[i]->model->name, groups[i].N_SIMS, groups[i].nodes,
+                                 groups[i].time_steps / 1000.0, now_s() - t0, left);
             }
+            if (!progressed && left > 0) std::this_thread::sleep_for(std::chrono::milliseconds(20));
         }
         return 0;
     } catch (const std::exception& e) {
