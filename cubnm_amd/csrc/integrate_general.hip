@@ -31,8 +31,7 @@ __device__ __forceinline__ void model_restart(const typename M::C& mc, double* s
 }  // namespace
 
 template <class M>
-__global__ void __launch_bounds__(1024) integrate_general_kernel(IntegrateArgs a, typename M::C mc) {
-    const int sim = blockIdx.x;
+__device__ __forceinline__ void general_body(const IntegrateArgs& a, const typename M::C& mc, const int sim) {
     const int j = threadIdx.x;
     const int N = a.nodes;
     const bool act = j < N;
@@ -197,18 +196,74 @@ __global__ void __launch_bounds__(1024) integrate_general_kernel(IntegrateArgs a
 }
 
 template <class M>
+__global__ void __launch_bounds__(1024) integrate_general_kernel(IntegrateArgs a, typename M::C mc) {
+    general_body<M>(a, mc, blockIdx.x);
+}
+
+// Several requests of one model in one launch: block b integrates simulation
+// blocks[b].y of request blocks[b].x, with that request's arguments and
+// model constants (durations, SCs, outputs and configs may all differ; the
+// launch shares the block size and the LDS layout).
+template <class M>
+__global__ void __launch_bounds__(1024) integrate_general_multi_kernel(const IntegrateArgs* args,
+                                                                       const typename M::C* consts,
+                                                                       const int2* blocks) {
+    const int2 b = blocks[blockIdx.x];
+    general_body<M>(args[b.x], consts[b.x], b.y);
+}
+
+size_t general_lds(const IntegrateArgs& a, int* sc_lds);
+
+template <class M>
 static hipError_t launch_general(const IntegrateArgs& a, const void* consts, hipStream_t st) {
     const typename M::C mc = *static_cast<const typename M::C*>(consts);
     int threads = ((a.nodes + 63) / 64) * 64;
     IntegrateArgs b = a;
-    size_t lds = sizeof(double) * (size_t)a.nodes;
+    const size_t lds = general_lds(a, &b.sc_lds);
+    hipLaunchKernelGGL(integrate_general_kernel<M>, dim3(a.N_SIMS), dim3(threads), lds, st, b, mc);
+    return hipGetLastError();
+}
+
+// LDS bytes and SC staging of a request on the general integrator.
+size_t general_lds(const IntegrateArgs& a, int* sc_lds) {
+    const size_t lds = sizeof(double) * (size_t)a.nodes;
     const size_t with_sc = lds + sizeof(double) * (size_t)a.nodes * (a.nodes + 1);
     // staged when it leaves room for two workgroups per CU, or when there are
     // too few simulations to fill the CUs anyway (latency-bound)
-    b.sc_lds = with_sc <= 64 * 1024 || (with_sc <= 150 * 1024 && a.N_SIMS <= 512);
-    if (b.sc_lds) lds = with_sc;
-    hipLaunchKernelGGL(integrate_general_kernel<M>, dim3(a.N_SIMS), dim3(threads), lds, st, b, mc);
+    *sc_lds = with_sc <= 64 * 1024 || (with_sc <= 150 * 1024 && a.N_SIMS <= 512);
+    return *sc_lds ? with_sc : lds;
+}
+
+template <class M>
+static hipError_t launch_general_multi(const IntegrateArgs* d_args, const void* d_consts, const int2* d_blocks,
+                                       int n_blocks, int threads, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL(integrate_general_multi_kernel<M>, dim3(n_blocks), dim3(threads), lds, st, d_args,
+                       static_cast<const typename M::C*>(d_consts), d_blocks);
     return hipGetLastError();
+}
+
+size_t model_consts_bytes(int model) {
+    switch (model) {
+        case MODEL_RWW: return sizeof(RWW::C);
+        case MODEL_JR: return sizeof(JR::C);
+        case MODEL_KURAMOTO: return sizeof(Kuramoto::C);
+        default: return sizeof(WC::C);
+    }
+}
+
+// d_args: [n_req] arguments (sc_lds set consistently with lds); d_consts:
+// [n_req] model constants of `model`; d_blocks: [n_blocks] (request, sim).
+hipError_t launch_integrate_general_multi(int model, const IntegrateArgs* d_args, const void* d_consts,
+                                          const int2* d_blocks, int n_blocks, int threads, size_t lds,
+                                          hipStream_t s) {
+    if (threads < 64 || threads > 1024 || n_blocks < 1) return hipErrorInvalidValue;
+    switch (model) {
+        case MODEL_RWW: return launch_general_multi<RWW>(d_args, d_consts, d_blocks, n_blocks, threads, lds, s);
+        case MODEL_JR: return launch_general_multi<JR>(d_args, d_consts, d_blocks, n_blocks, threads, lds, s);
+        case MODEL_KURAMOTO: return launch_general_multi<Kuramoto>(d_args, d_consts, d_blocks, n_blocks, threads, lds, s);
+        case MODEL_WC: return launch_general_multi<WC>(d_args, d_consts, d_blocks, n_blocks, threads, lds, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* consts, hipStream_t s) {

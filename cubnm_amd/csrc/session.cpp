@@ -128,6 +128,13 @@ struct NoiseEntry {
 
 std::map<NoiseKey, std::weak_ptr<NoiseEntry>> g_noise_cache;
 
+// Run the kernels on device-resident inputs already in s.d_* (or the caller's
+// device pointers), writing device outputs.
+struct DevIO {
+    const double* SC; const int* SCi; double* gp; double* rp;
+    double *bold, *fc, *fcd, *states; unsigned char* gb; int* gi;
+};
+
 // ---------------------------------------------------------------- session
 struct Session {
     const ModelInfo* model = nullptr;
@@ -167,6 +174,12 @@ struct Session {
     hipEvent_t ev[6] = {};
     double timings[6] = {0, 0, 0, 0, 0, 0};
     int last_integrator = -1;
+    int n_groups = 0;           // batched groups of the last request
+    IntegrateArgs iargs{};      // integrator arguments of a staged request
+    DevIO io{};                 // device buffers of a staged request
+    double* h_gp = nullptr;     // caller's parameter arrays (write-back target)
+    double* h_rp = nullptr;
+    double merged_ms = -1;      // integrator time when run in a merged launch
 };
 
 Session g_s;
@@ -486,12 +499,6 @@ void ensure_stream(Session& s) {
     for (auto& e : s.ev) check(hipEventCreate(&e), "hipEventCreate");
 }
 
-// Run the kernels on device-resident inputs already in s.d_* (or the caller's
-// device pointers), writing device outputs.
-struct DevIO {
-    const double* SC; const int* SCi; double* gp; double* rp;
-    double *bold, *fc, *fcd, *states; unsigned char* gb; int* gi;
-};
 
 // Groups of 16 simulations sharing one SC for the batched integrator: sims
 // are ordered by SC index (stable) and each SC's run is padded to a multiple
@@ -514,8 +521,11 @@ int build_groups(Session& s, const int* SCi_host) {
     return (int)(s.groups_h.size() / 16);
 }
 
-void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, const std::vector<int>* maxd,
-                 const int* SCi_host) {
+// Integrator arguments for the session's request; uploads delays, decides
+// batched vs per-simulation integrator and builds the batched groups and SC
+// fragments (queued on the session's stream).
+IntegrateArgs setup_integrate(Session& s, const DevIO& io, const std::vector<int>* delays,
+                              const std::vector<int>* maxd, const int* SCi_host) {
     const int N = s.nodes;
     IntegrateArgs a{};
     a.N_SIMS = s.N_SIMS; a.nodes = N;
@@ -561,22 +571,32 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
     if (g_force_integrator == 1 && !use_batched)
         throw std::runtime_error("batched integrator does not support this configuration (delay, numerical FIC, states_ts or node count)");
     s.last_integrator = use_batched ? 1 : 0;
-    int launches = 1;
-    int n_groups = 0;
+    s.n_groups = 0;
     if (use_batched) {
-        n_groups = build_groups(s, SCi_host);
+        s.n_groups = build_groups(s, SCi_host);
         int tpw, nt_pad;
         batched_plan(N, s.tpw, &tpw, &nt_pad);
         s.d_afrag.ensure(sizeof(double) * batched_afrag_doubles(N, nt_pad, s.N_SCs));
         check(build_afrag(io.SC, s.N_SCs, N, nt_pad, s.d_afrag.as<double>(), s.stream), "build_afrag launch");
     }
+    return a;
+}
+
+void launch_integrator(Session& s, const IntegrateArgs& a) {
+    int launches = 1;
     check(hipEventRecord(s.ev[2], s.stream), "event");
-    if (use_batched)
+    if (s.last_integrator == 1)
         check(launch_integrate_batched(s.model->id, a, model_consts(s), s.stream, &launches, s.d_groups.as<int>(),
-                                       n_groups, s.d_afrag.as<double>(), s.tpw),
+                                       s.n_groups, s.d_afrag.as<double>(), s.tpw),
               "integrate_batched launch");
     else check(launch_integrate_general(s.model->id, a, model_consts(s), s.stream), "integrate_general launch");
     check(hipEventRecord(s.ev[3], s.stream), "event");
+    s.timings[5] = launches;
+}
+
+// FC / FCD of the session's BOLD (queued on its stream).
+void launch_fc(Session& s, const DevIO& io) {
+    const int N = s.nodes;
     if (s.conf.do_fc) {
         FcArgs f{};
         f.N_SIMS = s.N_SIMS; f.nodes = N; f.bold_len = s.bold_len; f.n_vols_remove = s.n_vols_remove;
@@ -618,7 +638,13 @@ void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, co
         }
     }
     check(hipEventRecord(s.ev[4], s.stream), "event");
-    s.timings[5] = launches;
+}
+
+void run_kernels(Session& s, const DevIO& io, const std::vector<int>* delays, const std::vector<int>* maxd,
+                 const int* SCi_host) {
+    const IntegrateArgs a = setup_integrate(s, io, delays, maxd, SCi_host);
+    launch_integrator(s, a);
+    launch_fc(s, io);
 }
 
 void require_session() {
@@ -670,10 +696,10 @@ void prepare_session(Session& s, const char* model_name, int n_config, const cha
     dims->init_time = now_s() - t0;
 }
 
-// core.cpp:388-450 on host buffers, first half: host parameter preparation,
-// uploads, kernels and downloads queued on the session's stream.
-void launch_run(Session& s, const double* SC, const int* SC_indices, const double* SC_dist, double* gp, double* rp,
-                const double* v_list, cubnm_outputs* out) {
+// core.cpp:388-450 on host buffers, first part: host parameter preparation,
+// device buffers, uploads and integrator set-up queued on the session's stream.
+void stage_run(Session& s, const double* SC, const int* SC_indices, const double* SC_dist, double* gp, double* rp,
+               const double* v_list) {
     s.t_start = now_s();
     const int N = s.nodes, S = s.N_SIMS;
     const ModelInfo* m = s.model;
@@ -706,10 +732,23 @@ void launch_run(Session& s, const double* SC, const int* SC_indices, const doubl
     check(hipMemcpyAsync(s.d_gb.p, s.gb_host.data(), s.gb_host.size(), hipMemcpyHostToDevice, s.stream), "bools H2D");
     check(hipMemsetAsync(s.d_gi.p, 0, sizeof(int) * std::max<size_t>(1, (size_t)m->n_gi * S), s.stream), "memset");
     check(hipEventRecord(s.ev[1], s.stream), "event");
-    DevIO io{s.d_SC.as<double>(), s.d_SCi.as<int>(), s.d_gp.as<double>(), s.d_rp.as<double>(),
-             s.d_bold.as<double>(), s.d_fc.as<double>(), s.d_fcd.as<double>(), s.d_states.as<double>(),
-             s.d_gb.as<unsigned char>(), s.d_gi.as<int>()};
-    run_kernels(s, io, s.do_delay ? &s.delays_h : nullptr, s.do_delay ? &s.maxd_h : nullptr, SC_indices);
+    s.io = DevIO{s.d_SC.as<double>(), s.d_SCi.as<int>(), s.d_gp.as<double>(), s.d_rp.as<double>(),
+                 s.d_bold.as<double>(), s.d_fc.as<double>(), s.d_fcd.as<double>(), s.d_states.as<double>(),
+                 s.d_gb.as<unsigned char>(), s.d_gi.as<int>()};
+    s.iargs = setup_integrate(s, s.io, s.do_delay ? &s.delays_h : nullptr, s.do_delay ? &s.maxd_h : nullptr,
+                              SC_indices);
+    s.h_gp = gp; s.h_rp = rp;
+}
+
+// Last part: FC/FCD and downloads queued after the integrator.
+void queue_outputs(Session& s, cubnm_outputs* out) {
+    const int N = s.nodes, S = s.N_SIMS;
+    const ModelInfo* m = s.model;
+    launch_fc(s, s.io);
+    const long long st_stride = s.conf.states_ts ? (long long)s.states_len * N : N;
+    const size_t stb = sizeof(double) * (size_t)m->n_state * S * st_stride;
+    const size_t gpb = sizeof(double) * (size_t)m->n_global * S;
+    const size_t rpb = sizeof(double) * (size_t)m->n_regional * S * N;
     check(hipMemcpyAsync(out->sim_bold, s.d_bold.p, sizeof(double) * (size_t)S * s.bold_size, hipMemcpyDeviceToHost, s.stream), "BOLD D2H");
     if (s.conf.do_fc && out->sim_fc_trils)
         check(hipMemcpyAsync(out->sim_fc_trils, s.d_fc.p, sizeof(double) * (size_t)S * s.n_pairs, hipMemcpyDeviceToHost, s.stream), "FC D2H");
@@ -721,17 +760,24 @@ void launch_run(Session& s, const double* SC, const int* SC_indices, const doubl
     if (out->global_ints && m->n_gi > 0)
         check(hipMemcpyAsync(out->global_ints, s.d_gi.p, sizeof(int) * (size_t)m->n_gi * S, hipMemcpyDeviceToHost, s.stream), "ints D2H");
     if (s.modifies_params) {
-        check(hipMemcpyAsync(gp, s.d_gp.p, gpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
-        check(hipMemcpyAsync(rp, s.d_rp.p, rpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
+        check(hipMemcpyAsync(s.h_gp, s.d_gp.p, gpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
+        check(hipMemcpyAsync(s.h_rp, s.d_rp.p, rpb, hipMemcpyDeviceToHost, s.stream), "params D2H");
     }
     check(hipEventRecord(s.ev[5], s.stream), "event");
+}
+
+void launch_run(Session& s, const double* SC, const int* SC_indices, const double* SC_dist, double* gp, double* rp,
+                const double* v_list, cubnm_outputs* out) {
+    stage_run(s, SC, SC_indices, SC_dist, gp, rp, v_list);
+    launch_integrator(s, s.iargs);
+    queue_outputs(s, out);
 }
 
 // Second half: wait for the stream, timings, host-side noise outputs.
 void finish_run(Session& s, cubnm_outputs* out) {
     check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
     s.timings[1] = ev_ms(s.ev[0], s.ev[1]);
-    s.timings[2] = ev_ms(s.ev[2], s.ev[3]);
+    s.timings[2] = s.merged_ms >= 0 ? s.merged_ms : ev_ms(s.ev[2], s.ev[3]);
     s.timings[3] = ev_ms(s.ev[3], s.ev[4]);
     s.timings[4] = ev_ms(s.ev[4], s.ev[5]);
     if (s.noise_out) {
@@ -875,62 +921,145 @@ int cubnm_prepare_groups(int n_groups, cubnm_group* groups) {
 }
 
 int cubnm_run_groups(int n_groups, cubnm_group* groups) {
-    int launched_upto = 0;
+    int staged = 0;
     std::vector<int> order(std::max(0, n_groups));
     try {
         if (n_groups <= 0 || (int)g_pool.size() < n_groups) throw std::runtime_error("cubnm_prepare_groups first");
         for (int i = 0; i < n_groups; i++)
             if (!g_pool[i]->initialized || g_pool[i]->N_SIMS != groups[i].N_SIMS || g_pool[i]->nodes != groups[i].nodes)
                 throw std::runtime_error("group " + std::to_string(i) + " does not match its prepared session");
-        // costliest groups first: their kernels start while the host prepares the rest
+        bool verbose = false;
+        for (int i = 0; i < n_groups; i++) verbose = verbose || g_pool[i]->conf.verbose;
+        const double t0 = now_s();
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(),
                          [&](int a, int b) { return group_cost(groups[a]) > group_cost(groups[b]); });
-        for (; launched_upto < n_groups; launched_upto++) {
-            const int i = order[launched_upto];
+        // 1. host preparation, uploads and integrator set-up of every group
+        for (; staged < n_groups; staged++) {
+            const int i = order[staged];
             cubnm_group& g = groups[i];
             try {
-                launch_run(*g_pool[i], g.SC, g.SC_indices, g.SC_dist, g.global_params, g.regional_params, g.v_list,
-                           &g.out);
+                stage_run(*g_pool[i], g.SC, g.SC_indices, g.SC_dist, g.global_params, g.regional_params, g.v_list);
+            } catch (const std::exception& e) {
+                throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
+            }
+            g_pool[i]->merged_ms = -1;
+        }
+        for (int i = 0; i < n_groups; i++) check(hipStreamSynchronize(g_pool[i]->stream), "staging");
+        // 2. integrators: batched groups on their own streams; groups on the
+        //    per-simulation integrator merged into one launch per (model,
+        //    block size, LDS layout), longest simulations first
+        struct Bucket {
+            int model = 0, threads = 0;
+            size_t lds = 0;
+            std::vector<int> members;
+            hipEvent_t start = nullptr, stop = nullptr;
+            hipStream_t stream = nullptr;
+            bool done = false;
+        };
+        std::map<std::tuple<int, int, size_t>, Bucket> buckets;
+        std::vector<int> batched;
+        for (int k = 0; k < n_groups; k++) {
+            const int i = order[k];
+            Session& s = *g_pool[i];
+            if (s.last_integrator == 1) {
+                launch_integrator(s, s.iargs);
+                batched.push_back(i);
+                continue;
+            }
+            int sc_lds = 0;
+            const size_t lds = general_lds(s.iargs, &sc_lds);
+            s.iargs.sc_lds = sc_lds;
+            const int threads = ((s.nodes + 63) / 64) * 64;
+            Bucket& b = buckets[std::make_tuple(s.model->id, threads, lds)];
+            b.model = s.model->id; b.threads = threads; b.lds = lds;
+            b.members.push_back(i);
+        }
+        static std::vector<std::unique_ptr<DevBuf[]>> multi_bufs;  // kept: launches read them asynchronously
+        multi_bufs.clear();
+        for (auto& kv : buckets) {
+            Bucket& b = kv.second;
+            std::vector<IntegrateArgs> args;
+            const size_t cb = model_consts_bytes(b.model);
+            std::vector<char> consts;
+            std::vector<std::pair<long long, std::pair<int, int>>> blk;  // (cost, (request, sim))
+            for (size_t r = 0; r < b.members.size(); r++) {
+                Session& s = *g_pool[b.members[r]];
+                args.push_back(s.iargs);
+                const char* c = static_cast<const char*>(model_consts(s));
+                consts.insert(consts.end(), c, c + cb);
+                for (int sim = 0; sim < s.N_SIMS; sim++)
+                    blk.push_back({(long long)s.bw_it * s.inner_it, {(int)r, sim}});
+            }
+            std::stable_sort(blk.begin(), blk.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+            std::vector<int2> blocks;
+            for (const auto& e : blk) blocks.push_back(make_int2(e.second.first, e.second.second));
+            std::unique_ptr<DevBuf[]> bufs(new DevBuf[3]);
+            bufs[0].ensure(sizeof(IntegrateArgs) * args.size());
+            bufs[1].ensure(consts.size());
+            bufs[2].ensure(sizeof(int2) * blocks.size());
+            check(hipMemcpy(bufs[0].p, args.data(), sizeof(IntegrateArgs) * args.size(), hipMemcpyHostToDevice), "args H2D");
+            check(hipMemcpy(bufs[1].p, consts.data(), consts.size(), hipMemcpyHostToDevice), "consts H2D");
+            check(hipMemcpy(bufs[2].p, blocks.data(), sizeof(int2) * blocks.size(), hipMemcpyHostToDevice), "blocks H2D");
+            b.stream = g_pool[b.members[0]]->stream;
+            check(hipEventCreate(&b.start), "event");
+            check(hipEventCreate(&b.stop), "event");
+            check(hipEventRecord(b.start, b.stream), "event");
+            check(launch_integrate_general_multi(b.model, bufs[0].as<IntegrateArgs>(), bufs[1].p, bufs[2].as<int2>(),
+                                                 (int)blocks.size(), b.threads, b.lds, b.stream),
+                  "integrate_general_multi launch");
+            check(hipEventRecord(b.stop, b.stream), "event");
+            multi_bufs.push_back(std::move(bufs));
+            if (verbose)
+                std::fprintf(stderr, "cubnm_run_groups: %s, %zu requests, %zu simulations in one launch (%d threads)\n",
+                             g_pool[b.members[0]]->model->name, b.members.size(), blocks.size(), b.threads);
+        }
+        // 3. wait for the integrators, reporting each launch as it completes
+        for (size_t left = buckets.size() + batched.size(); left > 0;) {
+            bool progressed = false;
+            for (auto& kv : buckets) {
+                Bucket& b = kv.second;
+                if (b.done || hipEventQuery(b.stop) == hipErrorNotReady) continue;
+                b.done = true; left--; progressed = true;
+                const float ms = ev_ms(b.start, b.stop);
+                for (int i : b.members) g_pool[i]->merged_ms = ms;
+                if (verbose)
+                    std::fprintf(stderr, "cubnm_run_groups: %s launch of %zu requests done in %.1f s (at %.1f s)\n",
+                                 g_pool[b.members[0]]->model->name, b.members.size(), ms / 1000.0, now_s() - t0);
+            }
+            for (int& i : batched) {
+                if (i < 0 || hipEventQuery(g_pool[i]->ev[3]) == hipErrorNotReady) continue;
+                if (verbose)
+                    std::fprintf(stderr, "cubnm_run_groups: group %d (batched, %d x %d nodes) integrated at %.1f s\n",
+                                 i, groups[i].N_SIMS, groups[i].nodes, now_s() - t0);
+                i = -1; left--; progressed = true;
+            }
+            if (!progressed && left > 0) std::this_thread::sleep_for(std::chrono::milliseconds(20));
+        }
+        for (auto& kv : buckets) { hipEventDestroy(kv.second.start); hipEventDestroy(kv.second.stop); }
+        // 4. FC/FCD and downloads, then collect
+        for (int k = 0; k < n_groups; k++) {
+            const int i = order[k];
+            Session& s = *g_pool[i];
+            if (s.merged_ms >= 0) {
+                check(hipEventRecord(s.ev[2], s.stream), "event");
+                check(hipEventRecord(s.ev[3], s.stream), "event");
+            }
+            queue_outputs(s, &groups[i].out);
+        }
+        for (int k = 0; k < n_groups; k++) {
+            const int i = order[k];
+            try {
+                finish_run(*g_pool[i], &groups[i].out);
             } catch (const std::exception& e) {
                 throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
             }
         }
-        // collect groups as they complete; with "verbose" in any group's
-        // config, report each completion (sim-node-seconds and elapsed time)
-        bool verbose = false;
-        for (int i = 0; i < n_groups; i++) verbose = verbose || g_pool[i]->conf.verbose;
-        std::vector<char> done(n_groups, 0);
-        const double t0 = now_s();
-        for (int left = n_groups; left > 0;) {
-            bool progressed = false;
-            for (int k = 0; k < n_groups; k++) {
-                const int i = order[k];
-                if (done[i]) continue;
-                const hipError_t q = hipStreamQuery(g_pool[i]->stream);
-                if (q == hipErrorNotReady) continue;
-                try {
-                    finish_run(*g_pool[i], &groups[i].out);
-                } catch (const std::exception& e) {
-                    throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
-                }
-                done[i] = 1;
-                left--;
-                progressed = true;
-                if (verbose)
-                    std::fprintf(stderr, "cubnm_run_groups: group %d done (%s, %d sims x %d nodes x %.1f s) at %.1f s, %d left\n",
-                                 i, g_pool[i]->model->name, groups[i].N_SIMS, groups[i].nodes,
-                                 groups[i].time_steps / 1000.0, now_s() - t0, left);
-            }
-            if (!progressed && left > 0) std::this_thread::sleep_for(std::chrono::milliseconds(20));
-        }
+        if (verbose) std::fprintf(stderr, "cubnm_run_groups: all %d groups done at %.1f s\n", n_groups, now_s() - t0);
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();
-        for (int k = 0; k < launched_upto + 1 && k < n_groups; k++) {
-            Session& s = *g_pool[order[k]];
-            if (s.stream) hipStreamSynchronize(s.stream);
-        }
+        hipDeviceSynchronize();
         return -1;
     }
 }
