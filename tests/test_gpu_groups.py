@@ -1,9 +1,12 @@
 """Heterogeneous batches through one call (cubnm_run_groups / core.run_groups).
 
 Each request of a mixed batch (different models, node counts, durations,
-SCs, FIC settings and noise outputs) must give exactly what the same request
-gives alone through run_simulations (same kernels, same noise), which the
-parity tests tie to the oracle; one request is also checked against the
+SCs, FIC settings and noise outputs) must give what the same request gives
+alone through run_simulations: noise, shuffles and integer outputs exactly,
+BOLD / FC / FCD / states to 1e-12 (requests on the per-simulation integrator
+run in one merged launch whose kernel the compiler contracts into FMAs a
+little differently; observed 1.2e-14).  The parity tests tie
+run_simulations to the oracle; one request is also checked against the
 oracle directly at the usual 1e-9 tolerance.
 """
 import numpy as np
@@ -39,7 +42,16 @@ def test_groups_equal_single_requests():
         for k in single:
             if k in ("init_time", "run_time"):
                 continue
-            np.testing.assert_array_equal(g[k], single[k], err_msg=f"{r[0]} N={r[15]}: {k}")
+            msg = f"{r[0]} N={r[15]}: {k}"
+            if k == "sim_bold":
+                assert cases.bold_rel_err(g[k], single[k]).max() <= 1e-12, msg
+            elif k in ("sim_fc_trils", "sim_fcd_trils"):
+                assert cases.max_abs_err(g[k], single[k]) <= 1e-12, msg
+            elif k == "_sim_states":
+                scale = np.maximum(np.abs(single[k]).max(), 1e-300)
+                assert cases.max_abs_err(g[k] / scale, single[k] / scale) <= 1e-12, msg
+            else:
+                np.testing.assert_array_equal(g[k], single[k], err_msg=msg)
 
 
 def test_groups_match_oracle_and_write_back():
