@@ -3,9 +3,10 @@
 Each request of a mixed batch (different models, node counts, durations,
 SCs, FIC settings and noise outputs) must give what the same request gives
 alone through run_simulations: noise, shuffles and integer outputs exactly,
-BOLD / FC / FCD / states to 1e-12 (requests on the per-simulation integrator
-run in one merged launch whose kernel the compiler contracts into FMAs a
-little differently; observed 1.2e-14).  The parity tests tie
+BOLD and states to 1e-12, FC / FCD to 1e-10 (requests on the
+per-simulation integrator run in one merged launch whose kernel the compiler
+contracts into FMAs a little differently: BOLD 1.2e-14 observed, amplified
+to 8.7e-12 in the correlation between two 5-volume windows).  The parity tests tie
 run_simulations to the oracle; one request is also checked against the
 oracle directly at the usual 1e-9 tolerance.
 """
@@ -48,7 +49,7 @@ def test_groups_equal_single_requests():
             if k == "sim_bold":
                 assert cases.bold_rel_err(g[k], single[k]).max() <= 1e-12, msg
             elif k in ("sim_fc_trils", "sim_fcd_trils"):
-                assert cases.max_abs_err(g[k], single[k]) <= 1e-12, msg
+                assert cases.max_abs_err(g[k], single[k]) <= 1e-10, msg
             elif k == "_sim_states":
                 scale = np.maximum(np.abs(single[k]).max(), 1e-300)
                 assert cases.max_abs_err(g[k] / scale, single[k] / scale) <= 1e-12, msg
