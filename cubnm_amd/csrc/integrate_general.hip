@@ -36,6 +36,10 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
     const int N = a.nodes;
     const bool act = j < N;
     const int jj = act ? j : 0;  // inactive lanes mirror node 0 but never publish
+    // waves with no node (a launch shaped for larger requests) skip the work
+    // but keep meeting the barriers
+    const bool wact = (int)(threadIdx.x & ~63u) < N;
+    const int kN = wact ? N : 0;
     extern __shared__ double conn[];  // [N] previous-step coupling state (no delay), then SC when staged
     const double* SCg = a.SC + (long long)a.SC_indices[sim] * N * N;
     // SC in LDS (rows padded by one double) when it fits: the coupling loop
@@ -90,7 +94,7 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
             double g = 0.0;
             if (has_delay) {
                 const double sj = hist[(long long)buff_idx * N + jj];
-                for (int k = 0; k < N; k++) {
+                for (int k = 0; k < kN; k++) {
                     const int kb = (buff_idx + delay[(long long)k * N + jj]) % max_delay;
                     const double hk = hist[(long long)kb * N + k];
                     if constexpr (M::is_osc) g += SC[(long long)k * ldsc + jj] * sin(hk - sj);
@@ -99,7 +103,7 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
             } else {
                 const double sj = conn[jj];
 #pragma unroll 8
-                for (int k = 0; k < N; k++) {
+                for (int k = 0; k < kN; k++) {
                     if constexpr (M::is_osc) g += SC[(long long)k * ldsc + jj] * sin(conn[k] - sj);
                     else g += SC[(long long)k * ldsc + jj] * conn[k];
                 }
@@ -107,7 +111,7 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
             const long long nidx = ((long long)(sh_t * a.inner_it + inner_i) * N + sh_j) * M::n_noise;
             const double n0 = a.noise[nidx];
             const double n1 = (M::n_noise > 1) ? a.noise[nidx + 1] : 0.0;
-            M::step(mc, s, gp, rp, g, n0, n1);
+            if (wact) M::step(mc, s, gp, rp, g, n0, n1);
             __syncthreads();
             if (has_delay) {
                 if (act) hist[(long long)buff_idx * N + j] = s[M::conn_idx];
@@ -116,7 +120,7 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
                 conn[j] = s[M::conn_idx];
             }
         }
-        bw_step(a.bw, s[M::bold_idx], bx, bf, bn, bq);
+        if (wact) bw_step(a.bw, s[M::bold_idx], bx, bf, bn, bq);
         if ((bw_i + 1) % a.BOLD_TR_iters == 0) {
             if (act && bold_i < a.bold_len) bold[(long long)bold_i * N + j] = bold_signal(a.bw, bn, bq);
             bold_i++;

@@ -947,17 +947,18 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
         }
         for (int i = 0; i < n_groups; i++) check(hipStreamSynchronize(g_pool[i]->stream), "staging");
         // 2. integrators: batched groups on their own streams; groups on the
-        //    per-simulation integrator merged into one launch per (model,
-        //    block size, LDS layout), longest simulations first
+        //    per-simulation integrator merged into one launch per model (block
+        //    shaped for the largest node count; smaller requests leave the
+        //    extra waves idle), longest simulations first
         struct Bucket {
-            int model = 0, threads = 0;
+            int model = 0, threads = 64;
             size_t lds = 0;
             std::vector<int> members;
             hipEvent_t start = nullptr, stop = nullptr;
             hipStream_t stream = nullptr;
             bool done = false;
         };
-        std::map<std::tuple<int, int, size_t>, Bucket> buckets;
+        std::map<int, Bucket> buckets;
         std::vector<int> batched;
         for (int k = 0; k < n_groups; k++) {
             const int i = order[k];
@@ -967,52 +968,62 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
                 batched.push_back(i);
                 continue;
             }
-            int sc_lds = 0;
-            const size_t lds = general_lds(s.iargs, &sc_lds);
-            s.iargs.sc_lds = sc_lds;
-            const int threads = ((s.nodes + 63) / 64) * 64;
-            Bucket& b = buckets[std::make_tuple(s.model->id, threads, lds)];
-            b.model = s.model->id; b.threads = threads; b.lds = lds;
+            const size_t conn_b = sizeof(double) * (size_t)s.nodes;
+            const size_t with_sc = conn_b + sizeof(double) * (size_t)s.nodes * (s.nodes + 1);
+            s.iargs.sc_lds = with_sc <= 64 * 1024;   // small SCs staged; larger ones read through the caches
+            Bucket& b = buckets[s.model->id];
+            b.model = s.model->id;
+            b.threads = std::max(b.threads, ((s.nodes + 63) / 64) * 64);
+            b.lds = std::max(b.lds, s.iargs.sc_lds ? with_sc : conn_b);
             b.members.push_back(i);
         }
-        static std::vector<std::unique_ptr<DevBuf[]>> multi_bufs;  // kept: launches read them asynchronously
+        // host copies of the launch tables stay alive until the next call
+        static std::vector<std::vector<char>> host_tables;
+        static std::vector<std::unique_ptr<DevBuf[]>> multi_bufs;
+        host_tables.clear();
         multi_bufs.clear();
         for (auto& kv : buckets) {
             Bucket& b = kv.second;
-            std::vector<IntegrateArgs> args;
             const size_t cb = model_consts_bytes(b.model);
-            std::vector<char> consts;
-            std::vector<std::pair<long long, std::pair<int, int>>> blk;  // (cost, (request, sim))
+            std::vector<char> args, consts;
+            std::vector<std::pair<double, std::pair<int, int>>> blk;  // (cost, (request, sim))
             for (size_t r = 0; r < b.members.size(); r++) {
                 Session& s = *g_pool[b.members[r]];
-                args.push_back(s.iargs);
+                const char* ap = reinterpret_cast<const char*>(&s.iargs);
+                args.insert(args.end(), ap, ap + sizeof(IntegrateArgs));
                 const char* c = static_cast<const char*>(model_consts(s));
                 consts.insert(consts.end(), c, c + cb);
-                for (int sim = 0; sim < s.N_SIMS; sim++)
-                    blk.push_back({(long long)s.bw_it * s.inner_it, {(int)r, sim}});
+                const double cost = (double)s.bw_it * s.inner_it * s.nodes * (s.nodes + 40.0);
+                for (int sim = 0; sim < s.N_SIMS; sim++) blk.push_back({cost, {(int)r, sim}});
             }
             std::stable_sort(blk.begin(), blk.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-            std::vector<int2> blocks;
-            for (const auto& e : blk) blocks.push_back(make_int2(e.second.first, e.second.second));
+            std::vector<char> blocks(sizeof(int2) * blk.size());
+            for (size_t q = 0; q < blk.size(); q++) {
+                const int2 v = make_int2(blk[q].second.first, blk[q].second.second);
+                std::memcpy(blocks.data() + q * sizeof(int2), &v, sizeof(int2));
+            }
             std::unique_ptr<DevBuf[]> bufs(new DevBuf[3]);
-            bufs[0].ensure(sizeof(IntegrateArgs) * args.size());
+            bufs[0].ensure(args.size());
             bufs[1].ensure(consts.size());
-            bufs[2].ensure(sizeof(int2) * blocks.size());
-            check(hipMemcpy(bufs[0].p, args.data(), sizeof(IntegrateArgs) * args.size(), hipMemcpyHostToDevice), "args H2D");
-            check(hipMemcpy(bufs[1].p, consts.data(), consts.size(), hipMemcpyHostToDevice), "consts H2D");
-            check(hipMemcpy(bufs[2].p, blocks.data(), sizeof(int2) * blocks.size(), hipMemcpyHostToDevice), "blocks H2D");
+            bufs[2].ensure(blocks.size());
             b.stream = g_pool[b.members[0]]->stream;
+            check(hipMemcpyAsync(bufs[0].p, args.data(), args.size(), hipMemcpyHostToDevice, b.stream), "args H2D");
+            check(hipMemcpyAsync(bufs[1].p, consts.data(), consts.size(), hipMemcpyHostToDevice, b.stream), "consts H2D");
+            check(hipMemcpyAsync(bufs[2].p, blocks.data(), blocks.size(), hipMemcpyHostToDevice, b.stream), "blocks H2D");
             check(hipEventCreate(&b.start), "event");
             check(hipEventCreate(&b.stop), "event");
             check(hipEventRecord(b.start, b.stream), "event");
             check(launch_integrate_general_multi(b.model, bufs[0].as<IntegrateArgs>(), bufs[1].p, bufs[2].as<int2>(),
-                                                 (int)blocks.size(), b.threads, b.lds, b.stream),
+                                                 (int)blk.size(), b.threads, b.lds, b.stream),
                   "integrate_general_multi launch");
             check(hipEventRecord(b.stop, b.stream), "event");
+            host_tables.push_back(std::move(args));
+            host_tables.push_back(std::move(consts));
+            host_tables.push_back(std::move(blocks));
             multi_bufs.push_back(std::move(bufs));
             if (verbose)
-                std::fprintf(stderr, "cubnm_run_groups: %s, %zu requests, %zu simulations in one launch (%d threads)\n",
-                             g_pool[b.members[0]]->model->name, b.members.size(), blocks.size(), b.threads);
+                std::fprintf(stderr, "cubnm_run_groups: %s, %zu requests, %zu simulations in one launch (%d threads, %zu B LDS)\n",
+                             g_pool[b.members[0]]->model->name, b.members.size(), blk.size(), b.threads, b.lds);
         }
         // 3. wait for the integrators, reporting each launch as it completes
         for (size_t left = buckets.size() + batched.size(); left > 0;) {
