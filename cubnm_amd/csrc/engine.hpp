@@ -61,7 +61,7 @@ size_t general_lds(const IntegrateArgs& a, int* sc_lds);
 size_t model_consts_bytes(int model);
 hipError_t launch_integrate_general_multi(int model, const IntegrateArgs* d_args, const void* d_consts,
                                           const int2* d_blocks, int n_blocks, int threads, size_t lds,
-                                          hipStream_t s);
+                                          int* progress, hipStream_t s);
 hipError_t launch_fc_fcd(const FcArgs& a, hipStream_t s);
 // true when the per-simulation LDS kernel can hold a simulation's BOLD
 bool fc_fcd_fits_lds(const FcArgs& a);

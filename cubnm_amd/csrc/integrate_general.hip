@@ -211,9 +211,12 @@ __global__ void __launch_bounds__(1024) integrate_general_kernel(IntegrateArgs a
 template <class M>
 __global__ void __launch_bounds__(1024) integrate_general_multi_kernel(const IntegrateArgs* args,
                                                                        const typename M::C* consts,
-                                                                       const int2* blocks) {
+                                                                       const int2* blocks, int* progress) {
     const int2 b = blocks[blockIdx.x];
     general_body<M>(args[b.x], consts[b.x], b.y);
+    // completed simulations, readable by the host while the launch runs
+    if (threadIdx.x == 0 && progress)
+        __hip_atomic_fetch_add(progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 size_t general_lds(const IntegrateArgs& a, int* sc_lds);
@@ -240,9 +243,9 @@ size_t general_lds(const IntegrateArgs& a, int* sc_lds) {
 
 template <class M>
 static hipError_t launch_general_multi(const IntegrateArgs* d_args, const void* d_consts, const int2* d_blocks,
-                                       int n_blocks, int threads, size_t lds, hipStream_t st) {
+                                       int n_blocks, int threads, size_t lds, int* progress, hipStream_t st) {
     hipLaunchKernelGGL(integrate_general_multi_kernel<M>, dim3(n_blocks), dim3(threads), lds, st, d_args,
-                       static_cast<const typename M::C*>(d_consts), d_blocks);
+                       static_cast<const typename M::C*>(d_consts), d_blocks, progress);
     return hipGetLastError();
 }
 
@@ -256,16 +259,17 @@ size_t model_consts_bytes(int model) {
 }
 
 // d_args: [n_req] arguments (sc_lds set consistently with lds); d_consts:
-// [n_req] model constants of `model`; d_blocks: [n_blocks] (request, sim).
+// [n_req] model constants of `model`; d_blocks: [n_blocks] (request, sim);
+// progress: null, or a host-mapped counter of completed simulations.
 hipError_t launch_integrate_general_multi(int model, const IntegrateArgs* d_args, const void* d_consts,
                                           const int2* d_blocks, int n_blocks, int threads, size_t lds,
-                                          hipStream_t s) {
+                                          int* progress, hipStream_t s) {
     if (threads < 64 || threads > 1024 || n_blocks < 1) return hipErrorInvalidValue;
     switch (model) {
-        case MODEL_RWW: return launch_general_multi<RWW>(d_args, d_consts, d_blocks, n_blocks, threads, lds, s);
-        case MODEL_JR: return launch_general_multi<JR>(d_args, d_consts, d_blocks, n_blocks, threads, lds, s);
-        case MODEL_KURAMOTO: return launch_general_multi<Kuramoto>(d_args, d_consts, d_blocks, n_blocks, threads, lds, s);
-        case MODEL_WC: return launch_general_multi<WC>(d_args, d_consts, d_blocks, n_blocks, threads, lds, s);
+        case MODEL_RWW: return launch_general_multi<RWW>(d_args, d_consts, d_blocks, n_blocks, threads, lds, progress, s);
+        case MODEL_JR: return launch_general_multi<JR>(d_args, d_consts, d_blocks, n_blocks, threads, lds, progress, s);
+        case MODEL_KURAMOTO: return launch_general_multi<Kuramoto>(d_args, d_consts, d_blocks, n_blocks, threads, lds, progress, s);
+        case MODEL_WC: return launch_general_multi<WC>(d_args, d_consts, d_blocks, n_blocks, threads, lds, progress, s);
         default: return hipErrorInvalidValue;
     }
 }

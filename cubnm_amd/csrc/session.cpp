@@ -957,7 +957,18 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             hipEvent_t start = nullptr, stop = nullptr;
             hipStream_t stream = nullptr;
             bool done = false;
+            int n_sims = 0, reported = 0;
+            double last_report = 0;
         };
+        // host-mapped counters of completed simulations, one per model launch
+        static int* progress_host = nullptr;
+        static int* progress_dev = nullptr;
+        if (!progress_host) {
+            check(hipHostMalloc((void**)&progress_host, sizeof(int) * 8, hipHostMallocMapped | hipHostMallocCoherent),
+                  "hipHostMalloc");
+            check(hipHostGetDevicePointer((void**)&progress_dev, progress_host, 0), "hipHostGetDevicePointer");
+        }
+        for (int q = 0; q < 8; q++) progress_host[q] = 0;
         std::map<int, Bucket> buckets;
         std::vector<int> batched;
         for (int k = 0; k < n_groups; k++) {
@@ -1013,8 +1024,10 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             check(hipEventCreate(&b.start), "event");
             check(hipEventCreate(&b.stop), "event");
             check(hipEventRecord(b.start, b.stream), "event");
+            b.n_sims = (int)blk.size();
             check(launch_integrate_general_multi(b.model, bufs[0].as<IntegrateArgs>(), bufs[1].p, bufs[2].as<int2>(),
-                                                 (int)blk.size(), b.threads, b.lds, b.stream),
+                                                 (int)blk.size(), b.threads, b.lds, progress_dev + b.model,
+                                                 b.stream),
                   "integrate_general_multi launch");
             check(hipEventRecord(b.stop, b.stream), "event");
             host_tables.push_back(std::move(args));
@@ -1030,6 +1043,15 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             bool progressed = false;
             for (auto& kv : buckets) {
                 Bucket& b = kv.second;
+                if (!b.done && verbose) {
+                    const int c = __atomic_load_n(progress_host + b.model, __ATOMIC_RELAXED);
+                    if (c > b.reported && (now_s() - b.last_report > 10.0 || c == b.n_sims)) {
+                        std::fprintf(stderr, "cubnm_run_groups: %s launch: %d of %d simulations done at %.1f s\n",
+                                     g_pool[b.members[0]]->model->name, c, b.n_sims, now_s() - t0);
+                        b.reported = c;
+                        b.last_report = now_s();
+                    }
+                }
                 if (b.done || hipEventQuery(b.stop) == hipErrorNotReady) continue;
                 b.done = true; left--; progressed = true;
                 const float ms = ev_ms(b.start, b.stop);
