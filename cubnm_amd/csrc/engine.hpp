@@ -36,6 +36,7 @@ struct IntegrateArgs {
     long long hist_stride;
     BWConsts bw;
     int sc_lds;               // general kernel: SC staged in LDS (rows padded by one)
+    int ksplit;               // general kernel: partitions of the coupling sum over k (>= 1)
 };
 
 struct FcArgs {
@@ -56,8 +57,11 @@ struct FcArgs {
 
 // Host-side launchers (defined in the .hip files).
 hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* model_consts, hipStream_t s);
-// several requests of one model in one launch (see integrate_general.hip)
-size_t general_lds(const IntegrateArgs& a, int* sc_lds);
+// general kernel geometry: lanes per k partition, partitions (want 0 = auto),
+// LDS bytes of a workgroup; several requests of one model in one launch
+int general_np(int nodes);
+int general_ksplit(int nodes, int want);
+size_t general_lds_bytes(int model, int nodes, int np, int ksplit, bool sc_staged);
 size_t model_consts_bytes(int model);
 hipError_t launch_integrate_general_multi(int model, const IntegrateArgs* d_args, const void* d_consts,
                                           const int2* d_blocks, int n_blocks, int threads, size_t lds,

@@ -10,6 +10,14 @@
 // Per inner step: barrier, coupling g_j = sum_k SC[k][j] * s_k (Jacobi: all
 // lanes read the previous step's state from LDS), model step, barrier,
 // publish s_j.  BW runs once per bw step, BOLD every TR.
+//
+// The coupling sum is the hot loop: with one SC per simulation it streams
+// nodes^2 doubles per step from L2 / MALL / HBM.  It is split over `ksplit`
+// partitions of source nodes (more waves, more loads in flight; partial sums
+// added in partition order), its loads are typed as global (generic loads
+// would wait for each other) and issued eight at a time.  Oscillators
+// without delay use sin(x_k - x_j) = sin x_k cos x_j - cos x_k sin x_j: two
+// sums over published sin/cos instead of nodes^2 sine evaluations per step.
 #include "engine.hpp"
 
 namespace cubnm {
@@ -28,27 +36,80 @@ __device__ __forceinline__ void model_restart(const typename M::C& mc, double* s
     M::init(mc, s, rp);
 }
 
+// Global-memory view of a pointer whose address space the compiler cannot
+// infer (read from a table in memory, or chosen at run time between LDS and
+// global): generic (flat) loads are counted against both the vector-memory
+// and the LDS counters, so every load would wait for the previous one.
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ gmem<T>* as_global(T* p) { return (gmem<T>*)p; }
+
+// sum_{k0 <= k < k1} SC[k][j] * x_k (and the same with the second half of x
+// for oscillators: x = sin then cos), in ascending k.  SC is an LDS or a
+// global pointer (typed, so loads are issued back to back); x is in LDS.
+template <bool OSC, class P>
+__device__ __forceinline__ void coupling_sum(P sc, const int ldsc, const double* x, const int N, const int k0,
+                                             const int k1, double& g, double& g2) {
+    // batches of U loads issued back to back, then consumed in order (a
+    // software-pipelined form, loads of batch b+1 before batch b is consumed,
+    // spills at this kernel's 128-VGPR budget and is scheduled with a full
+    // wait at the loop head)
+    constexpr int U = 8;
+    int k = k0;
+    for (; k + U <= k1; k += U) {
+        double w[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) w[u] = sc[(long long)u * ldsc];
+        sc += (long long)U * ldsc;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            g += w[u] * x[k + u];
+            if constexpr (OSC) g2 += w[u] * x[N + k + u];
+        }
+    }
+    for (; k < k1; k++, sc += ldsc) {
+        const double w = *sc;
+        g += w * x[k];
+        if constexpr (OSC) g2 += w * x[N + k];
+    }
+}
+
 }  // namespace
 
 template <class M>
-__device__ __forceinline__ void general_body(const IntegrateArgs& a, const typename M::C& mc, const int sim) {
-    const int j = threadIdx.x;
+__device__ __forceinline__ void general_body(const IntegrateArgs& a, const typename M::C& mc, const int sim,
+                                             int* progress) {
+    // the block is ksplit partitions of np lanes (np a multiple of 64, so no
+    // wave straddles two partitions); every partition sums the coupling over
+    // its own range of source nodes k, partition 0 owns the node state and
+    // adds the other partitions' partial sums in partition order
+    const int ks = a.ksplit;
+    const int np = blockDim.x / ks;
+    const int part = threadIdx.x / np;
+    const int j = threadIdx.x - part * np;
     const int N = a.nodes;
-    const bool act = j < N;
-    const int jj = act ? j : 0;  // inactive lanes mirror node 0 but never publish
+    const bool own = part == 0;
+    const bool act = own && j < N;
+    const int jj = j < N ? j : 0;  // inactive lanes mirror node 0 but never publish
     // waves with no node (a launch shaped for larger requests) skip the work
     // but keep meeting the barriers
-    const bool wact = (int)(threadIdx.x & ~63u) < N;
-    const int kN = wact ? N : 0;
-    extern __shared__ double conn[];  // [N] previous-step coupling state (no delay), then SC when staged
-    const double* SCg = a.SC + (long long)a.SC_indices[sim] * N * N;
+    const bool wnode = (j & ~63) < N;
+    const bool wact = own && wnode;
+    const int kc = (N + ks - 1) / ks;
+    const int k0 = wnode ? min(N, part * kc) : 0;
+    const int k1 = wnode ? min(N, k0 + kc) : 0;
+    extern __shared__ double lds[];
+    double* red = lds;                       // [ks-1][2][np] partial sums of partitions 1..ks-1
+    double* conn = lds + (ks - 1) * 2 * np;  // previous-step coupling state: [N], or sin [N] then cos [N]
+    const int nc = M::is_osc ? 2 * N : N;
+    const gmem<const double>* SCg = as_global(a.SC) + (long long)a.SC_indices[sim] * N * N;
     // SC in LDS (rows padded by one double) when it fits: the coupling loop
     // then reads LDS instead of L1/L2 every step; summation order unchanged
-    double* scs = conn + N;
+    double* scs = conn + nc;
     const int ldsc = a.sc_lds ? N + 1 : N;
     if (a.sc_lds)
         for (int i = threadIdx.x; i < N * N; i += blockDim.x) scs[(i / N) * ldsc + i % N] = SCg[i];
-    const double* SC = a.sc_lds ? scs : SCg;
     double gp[M::n_global], rp[M::n_regional];
     for (int i = 0; i < M::n_global; i++) gp[i] = a.gp[(long long)i * a.N_SIMS + sim];
     for (int i = 0; i < M::n_regional; i++) rp[i] = a.rp[(long long)i * a.N_SIMS * N + (long long)sim * N + jj];
@@ -70,13 +131,19 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
 
     const int max_delay = a.do_delay ? a.max_delays[sim] : 0;
     const bool has_delay = max_delay > 0;
-    double* hist = has_delay ? a.hist + (long long)sim * a.hist_stride : nullptr;
-    const int* delay = has_delay ? a.delays + (long long)sim * N * N : nullptr;
+    gmem<double>* hist = has_delay ? as_global(a.hist) + (long long)sim * a.hist_stride : nullptr;
+    const gmem<const int>* delay = has_delay ? as_global(a.delays) + (long long)sim * N * N : nullptr;
     int buff_idx = max_delay - 1;
+    // oscillators without delay publish sin and cos of the phase:
+    // sum_k SC[k][j] sin(x_k - x_j) = cos x_j (SC^T sin x)_j - sin x_j (SC^T cos x)_j
+    auto publish = [&](double v) {
+        if constexpr (M::is_osc) sincos(v, &conn[j], &conn[N + j]);
+        else conn[j] = v;
+    };
     auto publish_all = [&]() {
         if (!act) return;
         if (has_delay) for (int bi = 0; bi < max_delay; bi++) hist[(long long)bi * N + j] = s[M::conn_idx];
-        else conn[j] = s[M::conn_idx];
+        else publish(s[M::conn_idx]);
     };
     publish_all();
 
@@ -84,46 +151,63 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
     double st_sum[M::n_state];
     for (int i = 0; i < M::n_state; i++) st_sum[i] = 0.0;
 
-    double* bold = a.bold + (long long)sim * a.bold_len * N;
+    gmem<double>* bold = as_global(a.bold) + (long long)sim * a.bold_len * N;
+    const gmem<const double>* noise = as_global(a.noise);
     int bw_i = 0, bold_i = 0, states_i = 0, rep = 0;
     while (bw_i < a.bw_it) {
-        const int sh_t = a.sh_ts[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it];
-        const int sh_j = a.sh_nodes[rep * N + jj];
+        const int sh_t = as_global(a.sh_ts)[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it];
+        const int sh_j = as_global(a.sh_nodes)[rep * N + jj];
         for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
             __syncthreads();
-            double g = 0.0;
+            double g = 0.0, g2 = 0.0;
             if (has_delay) {
                 const double sj = hist[(long long)buff_idx * N + jj];
-                for (int k = 0; k < kN; k++) {
+                for (int k = k0; k < k1; k++) {
                     const int kb = (buff_idx + delay[(long long)k * N + jj]) % max_delay;
                     const double hk = hist[(long long)kb * N + k];
-                    if constexpr (M::is_osc) g += SC[(long long)k * ldsc + jj] * sin(hk - sj);
-                    else g += SC[(long long)k * ldsc + jj] * hk;
+                    const double w = a.sc_lds ? scs[(long long)k * ldsc + jj] : SCg[(long long)k * ldsc + jj];
+                    if constexpr (M::is_osc) g += w * sin(hk - sj);
+                    else g += w * hk;
                 }
+            } else if (a.sc_lds) {
+                coupling_sum<M::is_osc>(scs + (long long)k0 * ldsc + jj, ldsc, conn, N, k0, k1, g, g2);
             } else {
-                const double sj = conn[jj];
-#pragma unroll 8
-                for (int k = 0; k < kN; k++) {
-                    if constexpr (M::is_osc) g += SC[(long long)k * ldsc + jj] * sin(conn[k] - sj);
-                    else g += SC[(long long)k * ldsc + jj] * conn[k];
+                coupling_sum<M::is_osc>(SCg + (long long)k0 * ldsc + jj, ldsc, conn, N, k0, k1, g, g2);
+            }
+            if (ks > 1) {
+                if (part > 0) {
+                    red[(long long)(part - 1) * 2 * np + j] = g;
+                    red[(long long)((part - 1) * 2 + 1) * np + j] = g2;
                 }
+                __syncthreads();
+                if (own)
+                    for (int p = 1; p < ks; p++) {
+                        g += red[(long long)(p - 1) * 2 * np + j];
+                        g2 += red[(long long)((p - 1) * 2 + 1) * np + j];
+                    }
+            }
+            if constexpr (M::is_osc) {
+                if (!has_delay) g = conn[N + jj] * g - conn[jj] * g2;
             }
             const long long nidx = ((long long)(sh_t * a.inner_it + inner_i) * N + sh_j) * M::n_noise;
-            const double n0 = a.noise[nidx];
-            const double n1 = (M::n_noise > 1) ? a.noise[nidx + 1] : 0.0;
+            const double n0 = wact ? noise[nidx] : 0.0;
+            const double n1 = (M::n_noise > 1 && wact) ? noise[nidx + 1] : 0.0;
             if (wact) M::step(mc, s, gp, rp, g, n0, n1);
             __syncthreads();
             if (has_delay) {
                 if (act) hist[(long long)buff_idx * N + j] = s[M::conn_idx];
                 buff_idx = (buff_idx + max_delay - 1) % max_delay;
             } else if (act) {
-                conn[j] = s[M::conn_idx];
+                publish(s[M::conn_idx]);
             }
         }
         if (wact) bw_step(a.bw, s[M::bold_idx], bx, bf, bn, bq);
         if ((bw_i + 1) % a.BOLD_TR_iters == 0) {
             if (act && bold_i < a.bold_len) bold[(long long)bold_i * N + j] = bold_signal(a.bw, bn, bq);
             bold_i++;
+            // volumes written, readable by the host while the launch runs (bnm.cu:362-370)
+            if (progress && threadIdx.x == 0)
+                __hip_atomic_fetch_add(progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         if ((bw_i + 1) % a.states_sampling_iters == 0) {
             if (a.states_ts) {
@@ -175,6 +259,8 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
             __syncthreads();
             model_restart<M>(mc, s, rp);
             if constexpr (M::n_gout_int > 0) fic_mean = 0.0;
+            if (progress && threadIdx.x == 0)  // the restarted volumes are written again (bnm.cu:443-446)
+                __hip_atomic_fetch_add(progress, -bold_i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             bx = 0.0; bf = 1.0; bn = 1.0; bq = 1.0;
             buff_idx = max_delay - 1;
             publish_all();
@@ -201,44 +287,68 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
 
 template <class M>
 __global__ void __launch_bounds__(1024) integrate_general_kernel(IntegrateArgs a, typename M::C mc) {
-    general_body<M>(a, mc, blockIdx.x);
+    general_body<M>(a, mc, blockIdx.x, nullptr);
 }
 
 // Several requests of one model in one launch: block b integrates simulation
 // blocks[b].y of request blocks[b].x, with that request's arguments and
 // model constants (durations, SCs, outputs and configs may all differ; the
-// launch shares the block size and the LDS layout).
+// launch shares the block size, the k partitions and the LDS size).
+// progress: null, or two host-mapped counters: [0] BOLD volumes written,
+// [1] simulations completed.
 template <class M>
 __global__ void __launch_bounds__(1024) integrate_general_multi_kernel(const IntegrateArgs* args,
                                                                        const typename M::C* consts,
                                                                        const int2* blocks, int* progress) {
     const int2 b = blocks[blockIdx.x];
-    general_body<M>(args[b.x], consts[b.x], b.y);
-    // completed simulations, readable by the host while the launch runs
+    general_body<M>(args[b.x], consts[b.x], b.y, progress);
     if (threadIdx.x == 0 && progress)
-        __hip_atomic_fetch_add(progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_fetch_add(progress + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-size_t general_lds(const IntegrateArgs& a, int* sc_lds);
+// LDS bytes of one general-kernel workgroup: partial sums of k partitions
+// 1..ksplit-1 (two per lane), the previous-step coupling state (sin and cos
+// for oscillators) and, when staged, SC with rows padded by one double.
+size_t general_lds_bytes(int model, int nodes, int np, int ksplit, bool sc_staged) {
+    size_t d = (size_t)(ksplit - 1) * 2 * np + (size_t)(model == MODEL_KURAMOTO ? 2 : 1) * nodes;
+    if (sc_staged) d += (size_t)nodes * (nodes + 1);
+    return d * sizeof(double);
+}
+
+// Lanes per k partition (nodes rounded up to whole waves) and the number of
+// partitions for `want` (0 = auto), at most what fits 1024 lanes.  Auto: about
+// 25 source nodes per partition, at most 4 (a partition's loads hide the
+// others' latency; each extra partition costs a barrier and an LDS exchange
+// per step).  Measured on MI355X, rWW with one SC per simulation, per step:
+// N=68 x 1024 sims (SC in LDS) 3.0 / 2.8 / 3.7 us with 1 / 2 / 4 partitions;
+// N=100 x 1024 10.6 / 10.6 / 8.2 us; N=100 x 1 3.6 / 2.3 / 1.7 us; N=200 and
+// N=400 within 5% of each other (bandwidth-bound: 7.2-7.6 TB/s of SC reads).
+int general_np(int nodes) { return ((nodes + 63) / 64) * 64; }
+int general_ksplit(int nodes, int want) {
+    const int kmax = std::max(1, std::min(8, 1024 / general_np(nodes)));
+    if (want > 0) return std::min(want, kmax);
+    return std::max(1, std::min({4, kmax, nodes / 25}));
+}
+
+// LDS bytes and SC staging of a request on the general integrator: staged
+// when it leaves room for two workgroups per CU, or when there are too few
+// simulations to fill the CUs anyway (latency-bound).
+static size_t general_lds(int model, const IntegrateArgs& a, int* sc_lds) {
+    const int np = general_np(a.nodes);
+    const size_t with_sc = general_lds_bytes(model, a.nodes, np, a.ksplit, true);
+    *sc_lds = with_sc <= 64 * 1024 || (with_sc <= 150 * 1024 && a.N_SIMS <= 512);
+    return *sc_lds ? with_sc : general_lds_bytes(model, a.nodes, np, a.ksplit, false);
+}
 
 template <class M>
-static hipError_t launch_general(const IntegrateArgs& a, const void* consts, hipStream_t st) {
+static hipError_t launch_general(int model, const IntegrateArgs& a, const void* consts, hipStream_t st) {
     const typename M::C mc = *static_cast<const typename M::C*>(consts);
-    int threads = ((a.nodes + 63) / 64) * 64;
     IntegrateArgs b = a;
-    const size_t lds = general_lds(a, &b.sc_lds);
+    if (b.ksplit < 1 || b.ksplit * general_np(a.nodes) > 1024) return hipErrorInvalidValue;
+    const int threads = b.ksplit * general_np(a.nodes);
+    const size_t lds = general_lds(model, a, &b.sc_lds);
     hipLaunchKernelGGL(integrate_general_kernel<M>, dim3(a.N_SIMS), dim3(threads), lds, st, b, mc);
     return hipGetLastError();
-}
-
-// LDS bytes and SC staging of a request on the general integrator.
-size_t general_lds(const IntegrateArgs& a, int* sc_lds) {
-    const size_t lds = sizeof(double) * (size_t)a.nodes;
-    const size_t with_sc = lds + sizeof(double) * (size_t)a.nodes * (a.nodes + 1);
-    // staged when it leaves room for two workgroups per CU, or when there are
-    // too few simulations to fill the CUs anyway (latency-bound)
-    *sc_lds = with_sc <= 64 * 1024 || (with_sc <= 150 * 1024 && a.N_SIMS <= 512);
-    return *sc_lds ? with_sc : lds;
 }
 
 template <class M>
@@ -258,9 +368,11 @@ size_t model_consts_bytes(int model) {
     }
 }
 
-// d_args: [n_req] arguments (sc_lds set consistently with lds); d_consts:
+// d_args: [n_req] arguments (sc_lds and ksplit set consistently with threads
+// and lds: threads = ksplit * general_np(largest node count)); d_consts:
 // [n_req] model constants of `model`; d_blocks: [n_blocks] (request, sim);
-// progress: null, or a host-mapped counter of completed simulations.
+// progress: null, or two host-mapped counters (volumes written, simulations
+// completed).
 hipError_t launch_integrate_general_multi(int model, const IntegrateArgs* d_args, const void* d_consts,
                                           const int2* d_blocks, int n_blocks, int threads, size_t lds,
                                           int* progress, hipStream_t s) {
@@ -277,10 +389,10 @@ hipError_t launch_integrate_general_multi(int model, const IntegrateArgs* d_args
 hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* consts, hipStream_t s) {
     if (a.nodes < 1 || a.nodes > 1024) return hipErrorInvalidValue;
     switch (model) {
-        case MODEL_RWW: return launch_general<RWW>(a, consts, s);
-        case MODEL_JR: return launch_general<JR>(a, consts, s);
-        case MODEL_KURAMOTO: return launch_general<Kuramoto>(a, consts, s);
-        case MODEL_WC: return launch_general<WC>(a, consts, s);
+        case MODEL_RWW: return launch_general<RWW>(model, a, consts, s);
+        case MODEL_JR: return launch_general<JR>(model, a, consts, s);
+        case MODEL_KURAMOTO: return launch_general<Kuramoto>(model, a, consts, s);
+        case MODEL_WC: return launch_general<WC>(model, a, consts, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -355,7 +355,8 @@ PyMethodDef methods[] = {
     {"last_timings", py_last_timings, METH_NOARGS, "last_timings() -> dict of phase timings of the last run (ms)"},
     {"force_integrator", py_force_integrator, METH_VARARGS, "force_integrator(k): -1 auto, 0 general, 1 batched"},
     {"set_tuning", py_set_tuning, METH_VARARGS,
-     "set_tuning(key, value): 'batched_tpw' 0/1/2; 'fc_mode' 0 auto, 1 FP64 Gram, 2 split-BF16 Gram"},
+     "set_tuning(key, value): 'batched_tpw' 0/1/2; 'general_ksplit' 0 auto or 1..8; "
+     "'fc_mode' 0 auto, 1 FP64 Gram, 2 split-BF16 Gram"},
     {nullptr, nullptr, 0, nullptr}};
 
 PyModuleDef module_def = {PyModuleDef_HEAD_INIT, "core", "MI355X simulation core (drop-in for cubnm.core)", -1, methods};

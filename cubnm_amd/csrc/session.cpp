@@ -186,6 +186,7 @@ Session g_s;
 std::vector<std::unique_ptr<Session>> g_pool;   // sessions of the grouped entry point
 int g_force_integrator = -1;  // -1 auto, 0 general, 1 batched
 int g_fc_mode = 0;            // 0 auto (LDS kernel when a simulation fits), 1 FP64 Gram, 2 split-BF16 Gram
+int g_general_ksplit = 0;     // partitions of the general kernel's coupling sum (0 auto)
 
 int parse_int(const std::string& v) { return std::stoi(v); }
 
@@ -555,6 +556,7 @@ IntegrateArgs setup_integrate(Session& s, const DevIO& io, const std::vector<int
         a.delays = s.d_delays.as<int>(); a.max_delays = s.d_maxd.as<int>(); a.hist = s.d_hist.as<double>();
     }
     a.do_delay = any_delay;
+    a.ksplit = general_ksplit(N, g_general_ksplit);
     const bool fic_restarts = s.model->id == MODEL_RWW && s.rww.do_fic && s.rww.max_fic_trials > 0;
     bool use_batched = !any_delay && !fic_restarts && !s.conf.states_ts && batched_supported(s.model->id, N);
     if (use_batched && g_force_integrator != 1) {
@@ -839,6 +841,11 @@ int cubnm_set_tuning(const char* key, int value) {
         g_batched_tpw = value;
         return 0;
     }
+    if (std::strcmp(key, "general_ksplit") == 0) {
+        if (value < 0 || value > 8) { g_err = "general_ksplit must be 0 (auto) or 1..8"; return -1; }
+        g_general_ksplit = value;
+        return 0;
+    }
     if (std::strcmp(key, "fc_mode") == 0) {
         if (value < 0 || value > 2) { g_err = "fc_mode must be 0 (auto), 1 (FP64 Gram) or 2 (split-BF16 Gram)"; return -1; }
         g_fc_mode = value;
@@ -951,16 +958,18 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
         //    shaped for the largest node count; smaller requests leave the
         //    extra waves idle), longest simulations first
         struct Bucket {
-            int model = 0, threads = 64;
+            int model = 0, np = 64, ks = 1, threads = 64;
             size_t lds = 0;
             std::vector<int> members;
             hipEvent_t start = nullptr, stop = nullptr;
             hipStream_t stream = nullptr;
             bool done = false;
-            int n_sims = 0, reported = 0;
+            int n_sims = 0, reported_vols = 0;
+            long long n_vols = 0;
             double last_report = 0;
         };
-        // host-mapped counters of completed simulations, one per model launch
+        // host-mapped progress counters, two per model launch: BOLD volumes
+        // written and simulations completed
         static int* progress_host = nullptr;
         static int* progress_dev = nullptr;
         if (!progress_host) {
@@ -979,14 +988,26 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
                 batched.push_back(i);
                 continue;
             }
-            const size_t conn_b = sizeof(double) * (size_t)s.nodes;
-            const size_t with_sc = conn_b + sizeof(double) * (size_t)s.nodes * (s.nodes + 1);
-            s.iargs.sc_lds = with_sc <= 64 * 1024;   // small SCs staged; larger ones read through the caches
             Bucket& b = buckets[s.model->id];
             b.model = s.model->id;
-            b.threads = std::max(b.threads, ((s.nodes + 63) / 64) * 64);
-            b.lds = std::max(b.lds, s.iargs.sc_lds ? with_sc : conn_b);
+            b.np = std::max(b.np, general_np(s.nodes));
+            b.n_vols += (long long)s.bold_len * s.N_SIMS;
             b.members.push_back(i);
+        }
+        for (auto& kv : buckets) {
+            Bucket& b = kv.second;
+            int nmax = 1;
+            for (int i : b.members) nmax = std::max(nmax, g_pool[i]->nodes);
+            b.ks = general_ksplit(nmax, g_general_ksplit);
+            b.threads = b.ks * b.np;
+            for (int i : b.members) {
+                Session& s = *g_pool[i];
+                // small SCs staged; larger ones read through the caches
+                const size_t with_sc = general_lds_bytes(b.model, s.nodes, b.np, b.ks, true);
+                s.iargs.sc_lds = with_sc <= 64 * 1024;
+                s.iargs.ksplit = b.ks;
+                b.lds = std::max(b.lds, general_lds_bytes(b.model, s.nodes, b.np, b.ks, s.iargs.sc_lds));
+            }
         }
         // host copies of the launch tables stay alive until the next call
         static std::vector<std::vector<char>> host_tables;
@@ -1026,7 +1047,7 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             check(hipEventRecord(b.start, b.stream), "event");
             b.n_sims = (int)blk.size();
             check(launch_integrate_general_multi(b.model, bufs[0].as<IntegrateArgs>(), bufs[1].p, bufs[2].as<int2>(),
-                                                 (int)blk.size(), b.threads, b.lds, progress_dev + b.model,
+                                                 (int)blk.size(), b.threads, b.lds, progress_dev + 2 * b.model,
                                                  b.stream),
                   "integrate_general_multi launch");
             check(hipEventRecord(b.stop, b.stream), "event");
@@ -1035,8 +1056,9 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             host_tables.push_back(std::move(blocks));
             multi_bufs.push_back(std::move(bufs));
             if (verbose)
-                std::fprintf(stderr, "cubnm_run_groups: %s, %zu requests, %zu simulations in one launch (%d threads, %zu B LDS)\n",
-                             g_pool[b.members[0]]->model->name, b.members.size(), blk.size(), b.threads, b.lds);
+                std::fprintf(stderr, "cubnm_run_groups: %s, %zu requests, %zu simulations in one launch "
+                             "(%d threads in %d k partitions, %zu B LDS)\n",
+                             g_pool[b.members[0]]->model->name, b.members.size(), blk.size(), b.threads, b.ks, b.lds);
         }
         // 3. wait for the integrators, reporting each launch as it completes
         for (size_t left = buckets.size() + batched.size(); left > 0;) {
@@ -1044,11 +1066,13 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             for (auto& kv : buckets) {
                 Bucket& b = kv.second;
                 if (!b.done && verbose) {
-                    const int c = __atomic_load_n(progress_host + b.model, __ATOMIC_RELAXED);
-                    if (c > b.reported && (now_s() - b.last_report > 10.0 || c == b.n_sims)) {
-                        std::fprintf(stderr, "cubnm_run_groups: %s launch: %d of %d simulations done at %.1f s\n",
-                                     g_pool[b.members[0]]->model->name, c, b.n_sims, now_s() - t0);
-                        b.reported = c;
+                    const int v = __atomic_load_n(progress_host + 2 * b.model, __ATOMIC_RELAXED);
+                    const int c = __atomic_load_n(progress_host + 2 * b.model + 1, __ATOMIC_RELAXED);
+                    if (v != b.reported_vols && (now_s() - b.last_report > 10.0 || c == b.n_sims)) {
+                        std::fprintf(stderr, "cubnm_run_groups: %s launch: %d of %d simulations done, %.1f%% of "
+                                     "volumes at %.1f s\n", g_pool[b.members[0]]->model->name, c, b.n_sims,
+                                     100.0 * v / std::max(1LL, b.n_vols), now_s() - t0);
+                        b.reported_vols = v;
                         b.last_report = now_s();
                     }
                 }

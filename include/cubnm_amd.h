@@ -140,6 +140,10 @@ int cubnm_last_integrator(void);
 /* Launch-shape and arithmetic knobs (no counterpart in the reference):
  *   "batched_tpw" = 16-node tiles per wave of the batched integrator
  *                   (0 = auto, 1, 2);
+ *   "general_ksplit" = partitions of the per-simulation integrator's
+ *                   coupling sum over source nodes (0 = auto, 1..8; capped
+ *                   at 1024 lanes per workgroup); the partial sums are added
+ *                   in partition order;
  *   "fc_mode"     = FC/FCD path: 0 = auto (per-simulation LDS kernel when a
  *                   simulation's BOLD fits in LDS, else FP64 Gram products),
  *                   1 = FP64 Gram products on MFMA, 2 = split-BF16 Gram

@@ -17,13 +17,18 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-9
 
 
-@pytest.fixture(scope="module", params=["general", "batched"])
+@pytest.fixture(scope="module", params=["general", "general_ks4", "batched"])
 def core(request):
+    """general_ks4: the per-simulation integrator with its coupling sum split
+    over 4 partitions of source nodes (partial sums added in partition order,
+    a different rounding order from the oracle's single sum)."""
     import cubnm_amd  # noqa: F401
     from cubnm_amd import _core
-    _core.force_integrator(0 if request.param == "general" else 1)
+    _core.force_integrator(1 if request.param == "batched" else 0)
+    _core.core.set_tuning("general_ksplit", 4 if request.param == "general_ks4" else 0)
     yield _core
     _core.force_integrator(-1)
+    _core.core.set_tuning("general_ksplit", 0)
 
 
 def _batched(core):
