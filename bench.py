@@ -109,17 +109,20 @@ def main():
     ap.add_argument("--tpw", type=int, default=0, help="batched integrator tiles per wave (0: auto)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (PMC byte-counter calibration)")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: the grid throughput (default); c3: heterogeneous batch; "
-                         "c4: gradient fit vs CMA-ES time-to-loss")
+                         "c4: gradient fit vs CMA-ES time-to-loss; c5: N=400 FC/FCD with split-BF16 Gram products")
     ap.add_argument("--c3-sims", type=int, default=4096, help="simulations of the heterogeneous batch (c3)")
     ap.add_argument("--c4-gens", type=int, default=60, help="CMA-ES generations (c4)")
     ap.add_argument("--c4-steps", type=int, default=40, help="maximum Adam steps (c4)")
     ap.add_argument("--c4-duration", type=float, default=120.0, help="simulated seconds (c4)")
+    ap.add_argument("--c5-sims", type=int, default=8192, help="simulations (c5)")
+    ap.add_argument("--c5-check-sims", type=int, default=4,
+                    help="simulations whose FC/FCD are recomputed in FP64 on the host (c5)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has one MI355X and the simulations shard "
-              "with no exchange (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: every box in this setup has one MI355X, and the simulations shard "
+              "across GPUs with no exchange between them; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -130,6 +133,8 @@ def main():
         return run_c4(args)
     if args.workload == "c3":
         return run_c3(args)
+    if args.workload == "c5":
+        return run_c5(args)
 
     dev = torch.device("cuda:0")
     S, N, dur = args.sims, args.nodes, args.duration
@@ -264,6 +269,7 @@ def run_c3(args):
     outs = _core.core.run_groups(reqs)
     wall = time.time() - t0
     assert len(outs) == len(reqs)
+    setup = sum(o["init_time"] for o in outs)  # host noise generation and session set-up
     line = {
         "metric": METRIC, "value": round(work / wall, 1), "unit": "sim-node-s/s", "n_gpus": 1, "steps": 1,
         "warmup": 0, "ms_per_step": round(1000 * wall, 1), "higher_is_better": True, "scaling": "weak",
@@ -271,7 +277,76 @@ def run_c3(args):
         "data": "synthetic (seeded lognormal SC per simulation, host-generated noise)",
         "config": {"workload": f"heterogeneous batch: {S} sims, N in {{68,100,200,400}}, 60-600 s, rWW/JR/Kuramoto "
                                f"70/15/15, own SC each, TR 0.72 s, FCD 30/5 TRs, {len(reqs)} requests in one call",
-                   "sim_node_seconds": work},
+                   "sim_node_seconds": work, "host_setup_s": round(setup, 2),
+                   "value_excluding_host_setup": round(work / (wall - setup), 1)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_c5(args):
+    """N=400 rWW (one shared seeded lognormal SC, G uniform in [0.5, 3],
+    other parameters at their defaults, FIC off), 300 s at dt 0.1 ms, TR 0.72 s
+    (416 volumes), FC and FCD over windows of 30 TRs with step 1 and no burn-in
+    (the reference's window placement, utils.cpp:21-72), FC/FCD as split-BF16
+    Gram products (fc_mode 2), integration FP64 on the per-simulation
+    integrator (the batched one stops at 256 nodes).  --c5-sims simulations
+    (8192 in the full configuration).  value = sim-node-seconds per wall
+    second of the whole run_simulations call.  For the first --c5-check-sims
+    simulations the FC and FCD are recomputed on the host in FP64 (NumPy
+    Pearson correlations of the returned BOLD, cubnm_amd.utils) and the
+    largest differences are reported."""
+    import torch  # noqa: F401
+    import cases
+    from cubnm_amd import _core, utils
+    N, dur, TR = 400, 300.0, 0.72
+    sc = cases.synthetic_sc(N, seed=0)
+    S = args.c5_sims
+    G = np.random.default_rng(5).uniform(0.5, 3.0, S)
+
+    def args_for(n, d):
+        gp, rp = cases.params("rWW", n, N, G=G[:n], do_fic=False, wIE=1.0)
+        cfg = cases.model_config("rWW", TR=TR, bold_remove_s=0, do_fic=False)
+        cfg.update({"window_size": "30", "window_step": "1", "verbose": "0"})
+        return cases.sim_args("rWW", sc, gp, rp, cfg, n, duration_s=d, TR_s=TR)
+
+    _core.core.set_tuning("fc_mode", 2)
+    try:
+        log(f"c5: warm-up (noise segment, workspaces): 16 simulations of 30 s")
+        _core.run_simulations(*args_for(16, 30.0))
+        log(f"c5: {S} simulations of {dur:g} s")
+        t0 = time.time()
+        out = _core.run_simulations(*args_for(S, dur))
+        wall = time.time() - t0
+        t = _core.last_timings()
+        assert t["fc_path"] == 2
+    finally:
+        _core.core.set_tuning("fc_mode", 0)
+    vols = out["sim_bold"].shape[1] // N
+    C = min(args.c5_check_sims, S)
+    fc_err = fcd_err = 0.0
+    for i in range(C):
+        b = out["sim_bold"][i].reshape(vols, N).T
+        fc_err = max(fc_err, float(np.abs(utils.calculate_fc(b) - out["sim_fc_trils"][i]).max()))
+        fcd_err = max(fcd_err, float(np.abs(utils.calculate_fcd(b, 30, 1) - out["sim_fcd_trils"][i]).max()))
+        log(f"c5: check {i + 1}/{C}: FC err {fc_err:.2e}, FCD err {fcd_err:.2e}")
+    n_w = int(round((1 + np.sqrt(1 + 8 * out["sim_fcd_trils"].shape[1])) / 2))
+    pairs = N * (N - 1) // 2
+    # FCD Gram products alone (3 BF16 terms): a lower bound on the FC/FCD stage's rate
+    fcd_flops = 2.0 * n_w * n_w * pairs * 3 * S
+    work = S * N * dur
+    line = {
+        "metric": METRIC, "value": round(work / wall, 1), "unit": "sim-node-s/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": round(1000 * wall, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 integration, split-bf16 FC/FCD products",
+        "data": "synthetic (seeded lognormal SC, host-generated noise)",
+        "config": {"workload": f"rWW N={N}, {S} sims, {dur:g} s, TR {TR} s ({vols} volumes), FC + FCD over "
+                               f"{n_w} windows of 30 TRs step 1, split-BF16 Gram products",
+                   "integrate_ms": round(t["integrate_ms"], 1), "fc_fcd_ms": round(t["fc_fcd_ms"], 1),
+                   "fc_fcd_sims_per_s": round(S / (t["fc_fcd_ms"] / 1000), 1),
+                   "fcd_bf16x3_tflops_lower_bound": round(fcd_flops / (t["fc_fcd_ms"] / 1000) / 1e12, 1),
+                   "check_sims": C, "fc_max_abs_err_vs_fp64_host": fc_err,
+                   "fcd_max_abs_err_vs_fp64_host": fcd_err},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

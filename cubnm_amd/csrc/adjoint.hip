@@ -52,10 +52,14 @@ struct AdjArgs {
     int sc_lds;                                     // SC staged in LDS (padded rows)
 };
 
-__device__ inline void noise_at(const AdjArgs& a, int bw_i, int inner_i, int j, double* n0, double* n1) {
+// Shuffled segment step and node of BW step bw_i (once per BW step), then the
+// two noise values of one inner step (the reference's index, bnm.cu:300-312).
+__device__ inline void noise_shuffle(const AdjArgs& a, int bw_i, int j, int* sh_t, int* sh_j) {
     const int rep = bw_i / a.noise_bw_it;
-    const int sh_t = a.sh_ts[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it];
-    const int sh_j = a.sh_nodes[rep * a.N + j];
+    *sh_t = a.sh_ts[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it];
+    *sh_j = a.sh_nodes[rep * a.N + j];
+}
+__device__ inline void noise_load(const AdjArgs& a, int sh_t, int sh_j, int inner_i, double* n0, double* n1) {
     const long long idx = ((long long)(sh_t * a.inner_it + inner_i) * a.N + sh_j) * 2;
     *n0 = a.noise[idx];
     *n1 = a.noise[idx + 1];
@@ -70,7 +74,11 @@ __device__ inline double dH(double a, double b, double d, double x) {
 }
 }  // namespace
 
-__global__ void __launch_bounds__(1024) rww_fwd_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
+// MAXT: launch bound (256 for up to 256 nodes, which lifts the register
+// budget from 128 to 256 VGPRs: the backward pass keeps one step of
+// prefetched scratch, noise and BW state in registers)
+template <int MAXT>
+__global__ void __launch_bounds__(MAXT) rww_fwd_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
     extern __shared__ double lds[];
     const int sim = blockIdx.x, j = threadIdx.x, N = a.N;
     const bool act = j < N;
@@ -95,8 +103,14 @@ __global__ void __launch_bounds__(1024) rww_fwd_kernel(AdjArgs a, RWW::C c, BWCo
             ck[0 * N + j] = s[4]; ck[1 * N + j] = s[5];
             ck[2 * N + j] = bx; ck[3 * N + j] = bf; ck[4 * N + j] = bn; ck[5 * N + j] = bq;
         }
+        int sh_t, sh_j;
+        noise_shuffle(a, bw_i, jj, &sh_t, &sh_j);
         for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
-            __syncthreads();
+            // this step's noise, issued before the barrier so that its latency
+            // overlaps the coupling sum (the barriers order LDS only)
+            double n0, n1;
+            noise_load(a, sh_t, sh_j, inner_i, &n0, &n1);
+            lds_barrier();
             double g = 0.0;
             if (a.sc_lds) {
 #pragma unroll 8
@@ -105,10 +119,8 @@ __global__ void __launch_bounds__(1024) rww_fwd_kernel(AdjArgs a, RWW::C c, BWCo
 #pragma unroll 8
                 for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
             }
-            double n0, n1;
-            noise_at(a, bw_i, inner_i, jj, &n0, &n1);
             RWW::step(c, s, gp, rp, g, n0, n1);
-            __syncthreads();
+            lds_barrier();
             if (act) conn[j] = s[4];
         }
         bw_step(bw, s[4], bx, bf, bn, bq);
@@ -119,7 +131,8 @@ __global__ void __launch_bounds__(1024) rww_fwd_kernel(AdjArgs a, RWW::C c, BWCo
     }
 }
 
-__global__ void __launch_bounds__(1024) rww_adj_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
+template <int MAXT>
+__global__ void __launch_bounds__(MAXT) rww_adj_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
     extern __shared__ double lds[];
     const int sim = blockIdx.x, j = threadIdx.x, N = a.N;
     const bool act = j < N;
@@ -154,8 +167,12 @@ __global__ void __launch_bounds__(1024) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
         if (act) conn[j] = s[4];
         for (int b = b0; b < b1; b++) {
             double* sb = scr + (long long)(b - b0) * per_bw * N;
+            int sh_t, sh_j;
+            noise_shuffle(a, b, jj, &sh_t, &sh_j);
             for (int i = 0; i < a.inner_it; i++) {
-                __syncthreads();
+                double n0, n1;
+                noise_load(a, sh_t, sh_j, i, &n0, &n1);
+                lds_barrier();
                 double g = 0.0;
                 if (a.sc_lds) {
 #pragma unroll 8
@@ -164,23 +181,44 @@ __global__ void __launch_bounds__(1024) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
 #pragma unroll 8
                     for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
                 }
+                // each lane reads back only its own scratch entries
                 if (act) { sb[(3 * i) * N + j] = s[4]; sb[(3 * i + 1) * N + j] = s[5]; sb[(3 * i + 2) * N + j] = g; }
-                double n0, n1;
-                noise_at(a, b, i, jj, &n0, &n1);
                 RWW::step(c, s, gp, rp, g, n0, n1);
-                __syncthreads();
+                lds_barrier();
                 if (act) conn[j] = s[4];
             }
             double* bwst = sb + (long long)3 * a.inner_it * N;
             if (act) { bwst[0 * N + j] = bx; bwst[1 * N + j] = bf; bwst[2 * N + j] = bn; bwst[3 * N + j] = bq; bwst[4 * N + j] = s[4]; }
             bw_step(bw, s[4], bx, bf, bn, bq);
         }
-        // ---- backwards through the segment
+        // ---- backwards through the segment.  The scratch entries and noise
+        // of the next inner step back are loaded one step ahead (pre*), and
+        // the BW-step state one BW step ahead (pb*), so their latency overlaps
+        // the current step; the barriers order LDS only.
+        int nsh_t, nsh_j;
+        noise_shuffle(a, b1 - 1, jj, &nsh_t, &nsh_j);
+        double preSE, preSI, preg, pren0, pren1;
+        {
+            const double* sb = scr + (long long)(b1 - 1 - b0) * per_bw * N;
+            const int i = a.inner_it - 1;
+            preSE = sb[(3 * i) * N + jj]; preSI = sb[(3 * i + 1) * N + jj]; preg = sb[(3 * i + 2) * N + jj];
+            noise_load(a, nsh_t, nsh_j, i, &pren0, &pren1);
+        }
+        double pbx, pbf, pbn, pbq, pbu;
+        {
+            const double* bwst = scr + (long long)(b1 - 1 - b0) * per_bw * N + (long long)3 * a.inner_it * N;
+            pbx = bwst[0 * N + jj]; pbf = bwst[1 * N + jj]; pbn = bwst[2 * N + jj]; pbq = bwst[3 * N + jj];
+            pbu = bwst[4 * N + jj];
+        }
         for (int b = b1 - 1; b >= b0; b--) {
-            const double* sb = scr + (long long)(b - b0) * per_bw * N;
-            const double* bwst = sb + (long long)3 * a.inner_it * N;
-            const double x = bwst[0 * N + jj], f = bwst[1 * N + jj], nu = bwst[2 * N + jj], q = bwst[3 * N + jj];
-            const double u = bwst[4 * N + jj];
+            const int sh_t = nsh_t, sh_j = nsh_j;
+            if (b > b0) noise_shuffle(a, b - 1, jj, &nsh_t, &nsh_j);
+            const double x = pbx, f = pbf, nu = pbn, q = pbq, u = pbu;
+            if (b > b0) {
+                const double* bwst = scr + (long long)(b - 1 - b0) * per_bw * N + (long long)3 * a.inner_it * N;
+                pbx = bwst[0 * N + jj]; pbf = bwst[1 * N + jj]; pbn = bwst[2 * N + jj]; pbq = bwst[3 * N + jj];
+                pbu = bwst[4 * N + jj];
+            }
             // post-BW state
             double x1 = x, f1 = f, nu1 = nu, q1 = q;
             bw_step(bw, u, x1, f1, nu1, q1);
@@ -205,11 +243,21 @@ __global__ void __launch_bounds__(1024) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
             const double lu = lx1 * a_dt;
             lx = lx_pre; lf = lf_pre; ln = ln_pre; lq = lq_pre;
             lSE += lu;
+            const double* sb = scr + (long long)(b - b0) * per_bw * N;
+            const double* sbn = scr + (long long)(b - 1 - b0) * per_bw * N;  // used only when b > b0
             // inner steps, last to first
             for (int i = a.inner_it - 1; i >= 0; i--) {
-                const double SE = sb[(3 * i) * N + jj], SI = sb[(3 * i + 1) * N + jj], g = sb[(3 * i + 2) * N + jj];
-                double n0, n1;
-                noise_at(a, b, i, jj, &n0, &n1);
+                const double SE = preSE, SI = preSI, g = preg, n0 = pren0, n1 = pren1;
+                if (i > 0) {
+                    preSE = sb[(3 * (i - 1)) * N + jj]; preSI = sb[(3 * (i - 1) + 1) * N + jj];
+                    preg = sb[(3 * (i - 1) + 2) * N + jj];
+                    noise_load(a, sh_t, sh_j, i - 1, &pren0, &pren1);
+                } else if (b > b0) {
+                    const int il = a.inner_it - 1;
+                    preSE = sbn[(3 * il) * N + jj]; preSI = sbn[(3 * il + 1) * N + jj];
+                    preg = sbn[(3 * il + 2) * N + jj];
+                    noise_load(a, nsh_t, nsh_j, il, &pren0, &pren1);
+                }
                 const double IE = c.w_E__I_0 + wp * JN * SE + g * G * JN - wIE * SI;
                 const double II = c.w_I__I_0 + JN * SE - c.w_II * SI;
                 const double aE = c.a_E * IE - c.b_E, aI = c.a_I * II - c.b_I;
@@ -223,9 +271,9 @@ __global__ void __launch_bounds__(1024) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
                 gGp += eE * JN * g;
                 gwp += eE * JN * SE;
                 gJN += eE * (wp * SE + G * g) + eI * SE;
-                __syncthreads();
+                lds_barrier();
                 if (act) ev[j] = eE * G * JN;
-                __syncthreads();
+                lds_barrier();
                 double coup = 0.0;
                 if (a.sc_lds) {
 #pragma unroll 8
@@ -322,7 +370,10 @@ int cubnm_rww_grad_forward(int n_sims, int nodes, int time_steps, int BOLD_TR, d
     const size_t with_sc = base + sizeof(double) * (size_t)nodes * (nodes + 1);
     A.sc_lds = with_sc <= 150 * 1024;
     const int threads = ((nodes + 63) / 64) * 64;
-    hipLaunchKernelGGL(rww_fwd_kernel, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
+    if (threads <= 256)
+        hipLaunchKernelGGL(rww_fwd_kernel<256>, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
+    else
+        hipLaunchKernelGGL(rww_fwd_kernel<1024>, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
     return finish(hipGetLastError());
 }
 
@@ -341,7 +392,10 @@ int cubnm_rww_grad_backward(int n_sims, int nodes, int time_steps, int BOLD_TR, 
     const size_t with_sc = base + sizeof(double) * (size_t)nodes * (nodes + 1);
     A.sc_lds = with_sc <= 150 * 1024;
     const int threads = ((nodes + 63) / 64) * 64;
-    hipLaunchKernelGGL(rww_adj_kernel, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
+    if (threads <= 256)
+        hipLaunchKernelGGL(rww_adj_kernel<256>, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
+    else
+        hipLaunchKernelGGL(rww_adj_kernel<1024>, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
     return finish(hipGetLastError());
 }
 

@@ -6,6 +6,19 @@
 
 namespace cubnm {
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() is a
+// workgroup fence on every address space, which on gfx9 waits for all
+// outstanding global loads and stores (vmcnt(0)) before the barrier; this
+// form waits only for LDS (lgkmcnt), so global loads issued before it (the
+// next step's noise, a prefetch) stay in flight across it.  Use it only where
+// the barrier protects LDS data: lanes that exchange data through global
+// memory need __syncthreads().
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Everything the integrator kernels read or write for one batch of
 // simulations. Layouts follow the reference boundary (src/ext/core.cpp:336-386):
 //   gp    [n_global][N_SIMS]            rp    [n_regional][N_SIMS*nodes]
@@ -63,7 +76,8 @@ int general_np(int nodes);
 int general_ksplit(int nodes, int want);
 size_t general_lds_bytes(int model, int nodes, int np, int ksplit, bool sc_staged);
 size_t model_consts_bytes(int model);
-hipError_t launch_integrate_general_multi(int model, const IntegrateArgs* d_args, const void* d_consts,
+size_t model_consts_stride();
+hipError_t launch_integrate_general_multi(const IntegrateArgs* d_args, const void* d_consts, const int* d_models,
                                           const int2* d_blocks, int n_blocks, int threads, size_t lds,
                                           int* progress, hipStream_t s);
 hipError_t launch_fc_fcd(const FcArgs& a, hipStream_t s);

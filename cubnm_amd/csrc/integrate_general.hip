@@ -158,7 +158,15 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
         const int sh_t = as_global(a.sh_ts)[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it];
         const int sh_j = as_global(a.sh_nodes)[rep * N + jj];
         for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
-            __syncthreads();
+            // this step's noise, issued before the barrier so that its latency
+            // overlaps the coupling sum; without delay the barriers order LDS
+            // only (with delay, lanes exchange state through the history in
+            // global memory and need the full barrier)
+            const long long nidx = ((long long)(sh_t * a.inner_it + inner_i) * N + sh_j) * M::n_noise;
+            const double n0 = wact ? noise[nidx] : 0.0;
+            const double n1 = (M::n_noise > 1 && wact) ? noise[nidx + 1] : 0.0;
+            if (has_delay) __syncthreads();
+            else lds_barrier();
             double g = 0.0, g2 = 0.0;
             if (has_delay) {
                 const double sj = hist[(long long)buff_idx * N + jj];
@@ -179,7 +187,7 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
                     red[(long long)(part - 1) * 2 * np + j] = g;
                     red[(long long)((part - 1) * 2 + 1) * np + j] = g2;
                 }
-                __syncthreads();
+                lds_barrier();
                 if (own)
                     for (int p = 1; p < ks; p++) {
                         g += red[(long long)(p - 1) * 2 * np + j];
@@ -189,11 +197,9 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
             if constexpr (M::is_osc) {
                 if (!has_delay) g = conn[N + jj] * g - conn[jj] * g2;
             }
-            const long long nidx = ((long long)(sh_t * a.inner_it + inner_i) * N + sh_j) * M::n_noise;
-            const double n0 = wact ? noise[nidx] : 0.0;
-            const double n1 = (M::n_noise > 1 && wact) ? noise[nidx + 1] : 0.0;
             if (wact) M::step(mc, s, gp, rp, g, n0, n1);
-            __syncthreads();
+            if (has_delay) __syncthreads();
+            else lds_barrier();
             if (has_delay) {
                 if (act) hist[(long long)buff_idx * N + j] = s[M::conn_idx];
                 buff_idx = (buff_idx + max_delay - 1) % max_delay;
@@ -290,18 +296,26 @@ __global__ void __launch_bounds__(1024) integrate_general_kernel(IntegrateArgs a
     general_body<M>(a, mc, blockIdx.x, nullptr);
 }
 
-// Several requests of one model in one launch: block b integrates simulation
-// blocks[b].y of request blocks[b].x, with that request's arguments and
-// model constants (durations, SCs, outputs and configs may all differ; the
-// launch shares the block size, the k partitions and the LDS size).
-// progress: null, or two host-mapped counters: [0] BOLD volumes written,
-// [1] simulations completed.
-template <class M>
-__global__ void __launch_bounds__(1024) integrate_general_multi_kernel(const IntegrateArgs* args,
-                                                                       const typename M::C* consts,
+// Every per-simulation request of a heterogeneous batch in one launch: block b
+// integrates simulation blocks[b].y of request blocks[b].x, with that
+// request's model (models[r]), arguments and model constants (consts + r *
+// cstride).  Durations, models, node counts, SCs, outputs and configs may all
+// differ; the launch shares the block size, the k partitions and the LDS
+// size.  progress: null, or two host-mapped counters: [0] BOLD volumes
+// written, [1] simulations completed.
+__global__ void __launch_bounds__(1024) integrate_general_multi_kernel(const IntegrateArgs* args, const char* consts,
+                                                                       int cstride, const int* models,
                                                                        const int2* blocks, int* progress) {
     const int2 b = blocks[blockIdx.x];
-    general_body<M>(args[b.x], consts[b.x], b.y, progress);
+    const char* c = consts + (long long)b.x * cstride;
+    switch (models[b.x]) {
+        case MODEL_RWW: general_body<RWW>(args[b.x], *reinterpret_cast<const RWW::C*>(c), b.y, progress); break;
+        case MODEL_JR: general_body<JR>(args[b.x], *reinterpret_cast<const JR::C*>(c), b.y, progress); break;
+        case MODEL_KURAMOTO:
+            general_body<Kuramoto>(args[b.x], *reinterpret_cast<const Kuramoto::C*>(c), b.y, progress);
+            break;
+        default: general_body<WC>(args[b.x], *reinterpret_cast<const WC::C*>(c), b.y, progress); break;
+    }
     if (threadIdx.x == 0 && progress)
         __hip_atomic_fetch_add(progress + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -351,14 +365,6 @@ static hipError_t launch_general(int model, const IntegrateArgs& a, const void* 
     return hipGetLastError();
 }
 
-template <class M>
-static hipError_t launch_general_multi(const IntegrateArgs* d_args, const void* d_consts, const int2* d_blocks,
-                                       int n_blocks, int threads, size_t lds, int* progress, hipStream_t st) {
-    hipLaunchKernelGGL(integrate_general_multi_kernel<M>, dim3(n_blocks), dim3(threads), lds, st, d_args,
-                       static_cast<const typename M::C*>(d_consts), d_blocks, progress);
-    return hipGetLastError();
-}
-
 size_t model_consts_bytes(int model) {
     switch (model) {
         case MODEL_RWW: return sizeof(RWW::C);
@@ -368,22 +374,26 @@ size_t model_consts_bytes(int model) {
     }
 }
 
+// Stride of the model-constants table of a merged launch: the largest model's
+// constants, rounded up to 16 bytes.
+size_t model_consts_stride() {
+    size_t m = 0;
+    for (int k : {MODEL_RWW, MODEL_JR, MODEL_KURAMOTO, MODEL_WC}) m = std::max(m, model_consts_bytes(k));
+    return (m + 15) & ~(size_t)15;
+}
+
 // d_args: [n_req] arguments (sc_lds and ksplit set consistently with threads
 // and lds: threads = ksplit * general_np(largest node count)); d_consts:
-// [n_req] model constants of `model`; d_blocks: [n_blocks] (request, sim);
-// progress: null, or two host-mapped counters (volumes written, simulations
-// completed).
-hipError_t launch_integrate_general_multi(int model, const IntegrateArgs* d_args, const void* d_consts,
+// [n_req] model constants at model_consts_stride(); d_models: [n_req] model
+// ids; d_blocks: [n_blocks] (request, sim); progress: null, or two
+// host-mapped counters (volumes written, simulations completed).
+hipError_t launch_integrate_general_multi(const IntegrateArgs* d_args, const void* d_consts, const int* d_models,
                                           const int2* d_blocks, int n_blocks, int threads, size_t lds,
                                           int* progress, hipStream_t s) {
     if (threads < 64 || threads > 1024 || n_blocks < 1) return hipErrorInvalidValue;
-    switch (model) {
-        case MODEL_RWW: return launch_general_multi<RWW>(d_args, d_consts, d_blocks, n_blocks, threads, lds, progress, s);
-        case MODEL_JR: return launch_general_multi<JR>(d_args, d_consts, d_blocks, n_blocks, threads, lds, progress, s);
-        case MODEL_KURAMOTO: return launch_general_multi<Kuramoto>(d_args, d_consts, d_blocks, n_blocks, threads, lds, progress, s);
-        case MODEL_WC: return launch_general_multi<WC>(d_args, d_consts, d_blocks, n_blocks, threads, lds, progress, s);
-        default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(integrate_general_multi_kernel, dim3(n_blocks), dim3(threads), lds, s, d_args,
+                       static_cast<const char*>(d_consts), (int)model_consts_stride(), d_models, d_blocks, progress);
+    return hipGetLastError();
 }
 
 hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* consts, hipStream_t s) {

@@ -953,12 +953,12 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             g_pool[i]->merged_ms = -1;
         }
         for (int i = 0; i < n_groups; i++) check(hipStreamSynchronize(g_pool[i]->stream), "staging");
-        // 2. integrators: batched groups on their own streams; groups on the
-        //    per-simulation integrator merged into one launch per model (block
-        //    shaped for the largest node count; smaller requests leave the
-        //    extra waves idle), longest simulations first
+        // 2. integrators: batched groups on their own streams; all groups on
+        //    the per-simulation integrator, whatever their model, merged into
+        //    one launch (block shaped for the largest node count; smaller
+        //    requests leave the extra waves idle), longest simulations first
         struct Bucket {
-            int model = 0, np = 64, ks = 1, threads = 64;
+            int np = 64, ks = 1, threads = 64;
             size_t lds = 0;
             std::vector<int> members;
             hipEvent_t start = nullptr, stop = nullptr;
@@ -968,7 +968,7 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             long long n_vols = 0;
             double last_report = 0;
         };
-        // host-mapped progress counters, two per model launch: BOLD volumes
+        // host-mapped progress counters of the merged launch: BOLD volumes
         // written and simulations completed
         static int* progress_host = nullptr;
         static int* progress_dev = nullptr;
@@ -988,8 +988,7 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
                 batched.push_back(i);
                 continue;
             }
-            Bucket& b = buckets[s.model->id];
-            b.model = s.model->id;
+            Bucket& b = buckets[0];
             b.np = std::max(b.np, general_np(s.nodes));
             b.n_vols += (long long)s.bold_len * s.N_SIMS;
             b.members.push_back(i);
@@ -1003,10 +1002,10 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             for (int i : b.members) {
                 Session& s = *g_pool[i];
                 // small SCs staged; larger ones read through the caches
-                const size_t with_sc = general_lds_bytes(b.model, s.nodes, b.np, b.ks, true);
+                const size_t with_sc = general_lds_bytes(s.model->id, s.nodes, b.np, b.ks, true);
                 s.iargs.sc_lds = with_sc <= 64 * 1024;
                 s.iargs.ksplit = b.ks;
-                b.lds = std::max(b.lds, general_lds_bytes(b.model, s.nodes, b.np, b.ks, s.iargs.sc_lds));
+                b.lds = std::max(b.lds, general_lds_bytes(s.model->id, s.nodes, b.np, b.ks, s.iargs.sc_lds));
             }
         }
         // host copies of the launch tables stay alive until the next call
@@ -1016,15 +1015,20 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
         multi_bufs.clear();
         for (auto& kv : buckets) {
             Bucket& b = kv.second;
-            const size_t cb = model_consts_bytes(b.model);
-            std::vector<char> args, consts;
+            const size_t cstride = model_consts_stride();
+            std::vector<char> args, consts, models;
             std::vector<std::pair<double, std::pair<int, int>>> blk;  // (cost, (request, sim))
             for (size_t r = 0; r < b.members.size(); r++) {
                 Session& s = *g_pool[b.members[r]];
                 const char* ap = reinterpret_cast<const char*>(&s.iargs);
                 args.insert(args.end(), ap, ap + sizeof(IntegrateArgs));
                 const char* c = static_cast<const char*>(model_consts(s));
+                const size_t cb = model_consts_bytes(s.model->id);
                 consts.insert(consts.end(), c, c + cb);
+                consts.resize(consts.size() + (cstride - cb), 0);
+                const int mid = s.model->id;
+                const char* mp = reinterpret_cast<const char*>(&mid);
+                models.insert(models.end(), mp, mp + sizeof(int));
                 const double cost = (double)s.bw_it * s.inner_it * s.nodes * (s.nodes + 40.0);
                 for (int sim = 0; sim < s.N_SIMS; sim++) blk.push_back({cost, {(int)r, sim}});
             }
@@ -1034,31 +1038,34 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
                 const int2 v = make_int2(blk[q].second.first, blk[q].second.second);
                 std::memcpy(blocks.data() + q * sizeof(int2), &v, sizeof(int2));
             }
-            std::unique_ptr<DevBuf[]> bufs(new DevBuf[3]);
+            std::unique_ptr<DevBuf[]> bufs(new DevBuf[4]);
             bufs[0].ensure(args.size());
             bufs[1].ensure(consts.size());
             bufs[2].ensure(blocks.size());
+            bufs[3].ensure(models.size());
             b.stream = g_pool[b.members[0]]->stream;
             check(hipMemcpyAsync(bufs[0].p, args.data(), args.size(), hipMemcpyHostToDevice, b.stream), "args H2D");
             check(hipMemcpyAsync(bufs[1].p, consts.data(), consts.size(), hipMemcpyHostToDevice, b.stream), "consts H2D");
             check(hipMemcpyAsync(bufs[2].p, blocks.data(), blocks.size(), hipMemcpyHostToDevice, b.stream), "blocks H2D");
+            check(hipMemcpyAsync(bufs[3].p, models.data(), models.size(), hipMemcpyHostToDevice, b.stream), "models H2D");
             check(hipEventCreate(&b.start), "event");
             check(hipEventCreate(&b.stop), "event");
             check(hipEventRecord(b.start, b.stream), "event");
             b.n_sims = (int)blk.size();
-            check(launch_integrate_general_multi(b.model, bufs[0].as<IntegrateArgs>(), bufs[1].p, bufs[2].as<int2>(),
-                                                 (int)blk.size(), b.threads, b.lds, progress_dev + 2 * b.model,
+            check(launch_integrate_general_multi(bufs[0].as<IntegrateArgs>(), bufs[1].p, bufs[3].as<int>(),
+                                                 bufs[2].as<int2>(), (int)blk.size(), b.threads, b.lds, progress_dev,
                                                  b.stream),
                   "integrate_general_multi launch");
             check(hipEventRecord(b.stop, b.stream), "event");
             host_tables.push_back(std::move(args));
             host_tables.push_back(std::move(consts));
             host_tables.push_back(std::move(blocks));
+            host_tables.push_back(std::move(models));
             multi_bufs.push_back(std::move(bufs));
             if (verbose)
-                std::fprintf(stderr, "cubnm_run_groups: %s, %zu requests, %zu simulations in one launch "
+                std::fprintf(stderr, "cubnm_run_groups: %zu requests, %zu simulations in one per-simulation launch "
                              "(%d threads in %d k partitions, %zu B LDS)\n",
-                             g_pool[b.members[0]]->model->name, b.members.size(), blk.size(), b.threads, b.ks, b.lds);
+                             b.members.size(), blk.size(), b.threads, b.ks, b.lds);
         }
         // 3. wait for the integrators, reporting each launch as it completes
         for (size_t left = buckets.size() + batched.size(); left > 0;) {
@@ -1066,11 +1073,11 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             for (auto& kv : buckets) {
                 Bucket& b = kv.second;
                 if (!b.done && verbose) {
-                    const int v = __atomic_load_n(progress_host + 2 * b.model, __ATOMIC_RELAXED);
-                    const int c = __atomic_load_n(progress_host + 2 * b.model + 1, __ATOMIC_RELAXED);
+                    const int v = __atomic_load_n(progress_host, __ATOMIC_RELAXED);
+                    const int c = __atomic_load_n(progress_host + 1, __ATOMIC_RELAXED);
                     if (v != b.reported_vols && (now_s() - b.last_report > 10.0 || c == b.n_sims)) {
-                        std::fprintf(stderr, "cubnm_run_groups: %s launch: %d of %d simulations done, %.1f%% of "
-                                     "volumes at %.1f s\n", g_pool[b.members[0]]->model->name, c, b.n_sims,
+                        std::fprintf(stderr, "cubnm_run_groups: per-simulation launch: %d of %d simulations done, "
+                                     "%.1f%% of volumes at %.1f s\n", c, b.n_sims,
                                      100.0 * v / std::max(1LL, b.n_vols), now_s() - t0);
                         b.reported_vols = v;
                         b.last_report = now_s();
@@ -1081,8 +1088,8 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
                 const float ms = ev_ms(b.start, b.stop);
                 for (int i : b.members) g_pool[i]->merged_ms = ms;
                 if (verbose)
-                    std::fprintf(stderr, "cubnm_run_groups: %s launch of %zu requests done in %.1f s (at %.1f s)\n",
-                                 g_pool[b.members[0]]->model->name, b.members.size(), ms / 1000.0, now_s() - t0);
+                    std::fprintf(stderr, "cubnm_run_groups: per-simulation launch of %zu requests done in %.1f s "
+                                 "(at %.1f s)\n", b.members.size(), ms / 1000.0, now_s() - t0);
             }
             for (int& i : batched) {
                 if (i < 0 || hipEventQuery(g_pool[i]->ev[3]) == hipErrorNotReady) continue;

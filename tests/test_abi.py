@@ -133,3 +133,14 @@ def test_run_groups_argument_errors():
     a[13] = True
     with pytest.raises(RuntimeError, match="use_cpu"):
         core.run_groups([tuple(a)])
+
+
+def test_tuning_keys_validated():
+    import cubnm_amd.core as core
+    for key, bad in [("batched_tpw", 3), ("general_ksplit", 9), ("general_ksplit", -1), ("fc_mode", 3)]:
+        with pytest.raises(ValueError):
+            core.set_tuning(key, bad)
+    with pytest.raises(ValueError, match="unknown tuning key"):
+        core.set_tuning("no_such_key", 0)
+    for key in ["batched_tpw", "general_ksplit", "fc_mode"]:
+        core.set_tuning(key, 0)
