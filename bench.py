@@ -62,26 +62,27 @@ def ops_per_sim_node_second(nodes, dt_ms, bw_dt_ms, TR_s):
     return (1000.0 / dt_ms) * (2 * nodes + 42) + (1000.0 / bw_dt_ms) * 24 + 10.0 / TR_s
 
 
-def cpu_baseline(n_cores, nodes, duration_s, G, W, J, sc, cfg):
+def cpu_baseline(n_cores, nodes, duration_s, G, W, J, sc, cfg, per_core=3):
     """The oracle (FP64 C++ restatement of the reference CPU engine, same
-    flags) on a bounded sample of the grid: one full-length simulation per
-    thread, run phase only."""
+    flags) on a bounded sample of the grid: per_core full-length simulations
+    per thread (about 10-15 s of CPU work on 16 cores), run phase only."""
     import cases
     from oracle import oracle as O
     O.lib()
-    pick = np.linspace(0, G.size - 1, n_cores).round().astype(int)
-    gp, rp = cases.params("rWW", n_cores, nodes, G=G[pick], w_p=W[pick][:, None], J_N=J[pick][:, None],
+    n = n_cores * per_core
+    pick = np.linspace(0, G.size - 1, n).round().astype(int)
+    gp, rp = cases.params("rWW", n, nodes, G=G[pick], w_p=W[pick][:, None], J_N=J[pick][:, None],
                           do_fic=False, wIE=1.0)
-    args = cases.sim_args("rWW", sc, gp, rp, cfg, n_cores, duration_s=duration_s)
+    args = cases.sim_args("rWW", sc, gp, rp, cfg, n, duration_s=duration_s)
     out = cases.run_oracle(args, n_threads=n_cores)
     rt = out["run_time"]
     return {
-        "value": n_cores * nodes * duration_s / rt,
+        "value": n * nodes * duration_s / rt,
         "unit": "sim-node-s/s",
         "cores": n_cores,
         "kind": "port",
-        "sample": f"{n_cores} of the grid's simulations (evenly spaced), full {duration_s:g} s each, "
-                  f"one per OpenMP thread; run phase only ({rt:.2f} s), noise generation excluded",
+        "sample": f"{n} of the grid's simulations (evenly spaced), full {duration_s:g} s each, "
+                  f"{per_core} per OpenMP thread; run phase only ({rt:.2f} s), noise generation excluded",
     }
 
 
@@ -332,8 +333,9 @@ def run_c5(args):
         log(f"c5: check {i + 1}/{C}: FC err {fc_err:.2e}, FCD err {fcd_err:.2e}")
     n_w = int(round((1 + np.sqrt(1 + 8 * out["sim_fcd_trils"].shape[1])) / 2))
     pairs = N * (N - 1) // 2
-    # FCD Gram products alone (3 BF16 terms): a lower bound on the FC/FCD stage's rate
-    fcd_flops = 2.0 * n_w * n_w * pairs * 3 * S
+    # FCD Gram products alone, lower triangle (the FCD tril), 3 BF16 terms: a
+    # lower bound on the FC/FCD stage's rate
+    fcd_flops = 2.0 * (n_w * (n_w - 1) // 2) * pairs * 3 * S
     work = S * N * dur
     line = {
         "metric": METRIC, "value": round(work / wall, 1), "unit": "sim-node-s/s", "n_gpus": 1, "steps": 1,

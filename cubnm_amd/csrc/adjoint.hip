@@ -50,6 +50,7 @@ struct AdjArgs {
     double* scratch;                                // [S][ck][inner_it*3 + 5][N]
     double *gG, *gwp, *gJN;                         // [S], [S][N], [S][N]
     int sc_lds;                                     // SC staged in LDS (padded rows)
+    int np, ks;                                     // lanes per k partition, partitions
 };
 
 // Shuffled segment step and node of BW step bw_i (once per BW step), then the
@@ -80,11 +81,17 @@ __device__ inline double dH(double a, double b, double d, double x) {
 template <int MAXT>
 __global__ void __launch_bounds__(MAXT) rww_fwd_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
     extern __shared__ double lds[];
-    const int sim = blockIdx.x, j = threadIdx.x, N = a.N;
-    const bool act = j < N;
-    const int jj = act ? j : 0;
-    double* conn = lds;               // [N]
-    double* scs = lds + N;            // [N][N+1] SC copy when a.sc_lds
+    const int sim = blockIdx.x, N = a.N, np = a.np, ks = a.ks;
+    // ks partitions of np lanes; partition p sums sources [k0, k1), partition 0
+    // owns the node state and adds the others' partial sums in partition order
+    const int part = threadIdx.x / np, j = threadIdx.x - part * np;
+    const bool own = part == 0;
+    const bool act = own && j < N;
+    const int jj = j < N ? j : 0;
+    const int kc = (N + ks - 1) / ks, k0 = min(N, part * kc), k1 = min(N, k0 + kc);
+    double* conn = lds;                   // [N]
+    double* redk = lds + N;               // [ks-1][np] partial sums
+    double* scs = redk + (ks - 1) * np;   // [N][N+1] SC copy when a.sc_lds
     const int ldsc = N + 1;
     if (a.sc_lds)
         for (int i = threadIdx.x; i < N * N; i += blockDim.x) scs[(i / N) * ldsc + i % N] = a.SC[i];
@@ -103,27 +110,33 @@ __global__ void __launch_bounds__(MAXT) rww_fwd_kernel(AdjArgs a, RWW::C c, BWCo
             ck[0 * N + j] = s[4]; ck[1 * N + j] = s[5];
             ck[2 * N + j] = bx; ck[3 * N + j] = bf; ck[4 * N + j] = bn; ck[5 * N + j] = bq;
         }
-        int sh_t, sh_j;
-        noise_shuffle(a, bw_i, jj, &sh_t, &sh_j);
+        int sh_t = 0, sh_j = 0;
+        if (own) noise_shuffle(a, bw_i, jj, &sh_t, &sh_j);
         for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
             // this step's noise, issued before the barrier so that its latency
             // overlaps the coupling sum (the barriers order LDS only)
-            double n0, n1;
-            noise_load(a, sh_t, sh_j, inner_i, &n0, &n1);
+            double n0 = 0.0, n1 = 0.0;
+            if (own) noise_load(a, sh_t, sh_j, inner_i, &n0, &n1);
             lds_barrier();
             double g = 0.0;
             if (a.sc_lds) {
 #pragma unroll 8
-                for (int k = 0; k < N; k++) g += scs[k * ldsc + jj] * conn[k];
+                for (int k = k0; k < k1; k++) g += scs[k * ldsc + jj] * conn[k];
             } else {
 #pragma unroll 8
-                for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
+                for (int k = k0; k < k1; k++) g += a.SC[(long long)k * N + jj] * conn[k];
             }
-            RWW::step(c, s, gp, rp, g, n0, n1);
+            if (ks > 1) {
+                if (!own) redk[(part - 1) * np + j] = g;
+                lds_barrier();
+                if (own)
+                    for (int p = 1; p < ks; p++) g += redk[(p - 1) * np + j];
+            }
+            if (own) RWW::step(c, s, gp, rp, g, n0, n1);
             lds_barrier();
             if (act) conn[j] = s[4];
         }
-        bw_step(bw, s[4], bx, bf, bn, bq);
+        if (own) bw_step(bw, s[4], bx, bf, bn, bq);
         if ((bw_i + 1) % a.TR_iters == 0) {
             if (act && bold_i < a.bold_len) bold[(long long)bold_i * N + j] = bold_signal(bw, bn, bq);
             bold_i++;
@@ -134,14 +147,21 @@ __global__ void __launch_bounds__(MAXT) rww_fwd_kernel(AdjArgs a, RWW::C c, BWCo
 template <int MAXT>
 __global__ void __launch_bounds__(MAXT) rww_adj_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
     extern __shared__ double lds[];
-    const int sim = blockIdx.x, j = threadIdx.x, N = a.N;
-    const bool act = j < N;
-    const int jj = act ? j : 0;
+    const int sim = blockIdx.x, N = a.N, np = a.np, ks = a.ks;
+    // k partitions as in the forward kernel: partition 0 owns the node state
+    // and the adjoint state; every partition sums its range of sources in the
+    // coupling (recompute) and in the coupling's transpose (backward)
+    const int part = threadIdx.x / np, j = threadIdx.x - part * np;
+    const bool own = part == 0;
+    const bool act = own && j < N;
+    const int jj = j < N ? j : 0;
+    const int kc = (N + ks - 1) / ks, k0 = min(N, part * kc), k1 = min(N, k0 + kc);
     const int ldsc = N + 1;
     double* conn = lds;                 // [N]
     double* ev = lds + N;               // [N]   e_j of the coupling adjoint
     double* red = lds + 2 * N;          // [16]  block reduction
-    double* scs = lds + 2 * N + 16;     // [N][N+1] SC copy when a.sc_lds
+    double* redk = red + 16;            // [ks-1][np] partial sums
+    double* scs = redk + (ks - 1) * np; // [N][N+1] SC copy when a.sc_lds
     if (a.sc_lds)
         for (int i = threadIdx.x; i < N * N; i += blockDim.x) scs[(i / N) * ldsc + i % N] = a.SC[i];
     const double G = a.G[sim];
@@ -161,129 +181,151 @@ __global__ void __launch_bounds__(MAXT) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
         const int b0 = seg * a.ck, b1 = min(a.bw_it, b0 + a.ck);
         // ---- recompute the segment from its checkpoint
         const double* ck = a.ckpt + ((long long)sim * a.n_ck + seg) * 6 * N;
-        double s[RWW::n_state] = {0, 0, 0, 0, ck[0 * N + jj], ck[1 * N + jj]};
-        double bx = ck[2 * N + jj], bf = ck[3 * N + jj], bn = ck[4 * N + jj], bq = ck[5 * N + jj];
+        double s[RWW::n_state] = {0, 0, 0, 0, 0, 0};
+        double bx = 0, bf = 1, bn = 1, bq = 1;
+        if (own) {
+            s[4] = ck[0 * N + jj]; s[5] = ck[1 * N + jj];
+            bx = ck[2 * N + jj]; bf = ck[3 * N + jj]; bn = ck[4 * N + jj]; bq = ck[5 * N + jj];
+        }
         __syncthreads();
         if (act) conn[j] = s[4];
         for (int b = b0; b < b1; b++) {
             double* sb = scr + (long long)(b - b0) * per_bw * N;
-            int sh_t, sh_j;
-            noise_shuffle(a, b, jj, &sh_t, &sh_j);
+            int sh_t = 0, sh_j = 0;
+            if (own) noise_shuffle(a, b, jj, &sh_t, &sh_j);
             for (int i = 0; i < a.inner_it; i++) {
-                double n0, n1;
-                noise_load(a, sh_t, sh_j, i, &n0, &n1);
+                double n0 = 0.0, n1 = 0.0;
+                if (own) noise_load(a, sh_t, sh_j, i, &n0, &n1);
                 lds_barrier();
                 double g = 0.0;
                 if (a.sc_lds) {
 #pragma unroll 8
-                    for (int k = 0; k < N; k++) g += scs[k * ldsc + jj] * conn[k];
+                    for (int k = k0; k < k1; k++) g += scs[k * ldsc + jj] * conn[k];
                 } else {
 #pragma unroll 8
-                    for (int k = 0; k < N; k++) g += a.SC[(long long)k * N + jj] * conn[k];
+                    for (int k = k0; k < k1; k++) g += a.SC[(long long)k * N + jj] * conn[k];
+                }
+                if (ks > 1) {
+                    if (!own) redk[(part - 1) * np + j] = g;
+                    lds_barrier();
+                    if (own)
+                        for (int p = 1; p < ks; p++) g += redk[(p - 1) * np + j];
                 }
                 // each lane reads back only its own scratch entries
                 if (act) { sb[(3 * i) * N + j] = s[4]; sb[(3 * i + 1) * N + j] = s[5]; sb[(3 * i + 2) * N + j] = g; }
-                RWW::step(c, s, gp, rp, g, n0, n1);
+                if (own) RWW::step(c, s, gp, rp, g, n0, n1);
                 lds_barrier();
                 if (act) conn[j] = s[4];
             }
             double* bwst = sb + (long long)3 * a.inner_it * N;
             if (act) { bwst[0 * N + j] = bx; bwst[1 * N + j] = bf; bwst[2 * N + j] = bn; bwst[3 * N + j] = bq; bwst[4 * N + j] = s[4]; }
-            bw_step(bw, s[4], bx, bf, bn, bq);
+            if (own) bw_step(bw, s[4], bx, bf, bn, bq);
         }
         // ---- backwards through the segment.  The scratch entries and noise
         // of the next inner step back are loaded one step ahead (pre*), and
         // the BW-step state one BW step ahead (pb*), so their latency overlaps
         // the current step; the barriers order LDS only.
-        int nsh_t, nsh_j;
-        noise_shuffle(a, b1 - 1, jj, &nsh_t, &nsh_j);
-        double preSE, preSI, preg, pren0, pren1;
-        {
+        int nsh_t = 0, nsh_j = 0;
+        double preSE = 0, preSI = 0, preg = 0, pren0 = 0, pren1 = 0;
+        double pbx = 0, pbf = 1, pbn = 1, pbq = 1, pbu = 0;
+        if (own) {
+            noise_shuffle(a, b1 - 1, jj, &nsh_t, &nsh_j);
             const double* sb = scr + (long long)(b1 - 1 - b0) * per_bw * N;
             const int i = a.inner_it - 1;
             preSE = sb[(3 * i) * N + jj]; preSI = sb[(3 * i + 1) * N + jj]; preg = sb[(3 * i + 2) * N + jj];
             noise_load(a, nsh_t, nsh_j, i, &pren0, &pren1);
-        }
-        double pbx, pbf, pbn, pbq, pbu;
-        {
-            const double* bwst = scr + (long long)(b1 - 1 - b0) * per_bw * N + (long long)3 * a.inner_it * N;
+            const double* bwst = sb + (long long)3 * a.inner_it * N;
             pbx = bwst[0 * N + jj]; pbf = bwst[1 * N + jj]; pbn = bwst[2 * N + jj]; pbq = bwst[3 * N + jj];
             pbu = bwst[4 * N + jj];
         }
         for (int b = b1 - 1; b >= b0; b--) {
             const int sh_t = nsh_t, sh_j = nsh_j;
-            if (b > b0) noise_shuffle(a, b - 1, jj, &nsh_t, &nsh_j);
-            const double x = pbx, f = pbf, nu = pbn, q = pbq, u = pbu;
-            if (b > b0) {
-                const double* bwst = scr + (long long)(b - 1 - b0) * per_bw * N + (long long)3 * a.inner_it * N;
-                pbx = bwst[0 * N + jj]; pbf = bwst[1 * N + jj]; pbn = bwst[2 * N + jj]; pbq = bwst[3 * N + jj];
-                pbu = bwst[4 * N + jj];
-            }
-            // post-BW state
-            double x1 = x, f1 = f, nu1 = nu, q1 = q;
-            bw_step(bw, u, x1, f1, nu1, q1);
-            if ((b + 1) % a.TR_iters == 0) {
-                const int vol = (b + 1) / a.TR_iters - 1;
-                if (vol < a.bold_len) {
-                    const double dy = dbold[(long long)vol * N + jj];
-                    lq += dy * bw.V_0 * (-bw.k1 - bw.k2 / nu1);
-                    ln += dy * bw.V_0 * (bw.k2 * q1 / (nu1 * nu1) - bw.k3);
-                }
-            }
-            // BW step adjoint (lx, lf, ln, lq refer to the post-step state here)
-            const double nu1a = pow(nu1, ra), nua = pow(nu, ra), r1f = pow(r1, 1.0 / f);
-            const double lq_pre = lq * (1.0 - a_it * nu1a / nu1);
-            const double ln1 = ln + lq * (-a_it * q * (ra - 1.0) * nu1a / (nu1 * nu1));
-            double lf_pre = lq * a_it / bw.rho * (1.0 - r1f + r1f * lnr1 / f);
-            const double ln_pre = ln1 * (1.0 - a_it * ra * nua / nu);
-            lf_pre += ln1 * a_it;
-            const double lx1 = lx + lf * a_dt;
-            lf_pre += lf - lx1 * a_dt * bw.y;
-            const double lx_pre = lx1 * (1.0 - a_dt * bw.kappa);
-            const double lu = lx1 * a_dt;
-            lx = lx_pre; lf = lf_pre; ln = ln_pre; lq = lq_pre;
-            lSE += lu;
             const double* sb = scr + (long long)(b - b0) * per_bw * N;
             const double* sbn = scr + (long long)(b - 1 - b0) * per_bw * N;  // used only when b > b0
+            if (own) {
+                if (b > b0) noise_shuffle(a, b - 1, jj, &nsh_t, &nsh_j);
+                const double x = pbx, f = pbf, nu = pbn, q = pbq, u = pbu;
+                if (b > b0) {
+                    const double* bwst = sbn + (long long)3 * a.inner_it * N;
+                    pbx = bwst[0 * N + jj]; pbf = bwst[1 * N + jj]; pbn = bwst[2 * N + jj]; pbq = bwst[3 * N + jj];
+                    pbu = bwst[4 * N + jj];
+                }
+                // post-BW state
+                double x1 = x, f1 = f, nu1 = nu, q1 = q;
+                bw_step(bw, u, x1, f1, nu1, q1);
+                if ((b + 1) % a.TR_iters == 0) {
+                    const int vol = (b + 1) / a.TR_iters - 1;
+                    if (vol < a.bold_len) {
+                        const double dy = dbold[(long long)vol * N + jj];
+                        lq += dy * bw.V_0 * (-bw.k1 - bw.k2 / nu1);
+                        ln += dy * bw.V_0 * (bw.k2 * q1 / (nu1 * nu1) - bw.k3);
+                    }
+                }
+                // BW step adjoint (lx, lf, ln, lq refer to the post-step state here)
+                const double nu1a = pow(nu1, ra), nua = pow(nu, ra), r1f = pow(r1, 1.0 / f);
+                const double lq_pre = lq * (1.0 - a_it * nu1a / nu1);
+                const double ln1 = ln + lq * (-a_it * q * (ra - 1.0) * nu1a / (nu1 * nu1));
+                double lf_pre = lq * a_it / bw.rho * (1.0 - r1f + r1f * lnr1 / f);
+                const double ln_pre = ln1 * (1.0 - a_it * ra * nua / nu);
+                lf_pre += ln1 * a_it;
+                const double lx1 = lx + lf * a_dt;
+                lf_pre += lf - lx1 * a_dt * bw.y;
+                const double lx_pre = lx1 * (1.0 - a_dt * bw.kappa);
+                const double lu = lx1 * a_dt;
+                lx = lx_pre; lf = lf_pre; ln = ln_pre; lq = lq_pre;
+                lSE += lu;
+            }
             // inner steps, last to first
             for (int i = a.inner_it - 1; i >= 0; i--) {
-                const double SE = preSE, SI = preSI, g = preg, n0 = pren0, n1 = pren1;
-                if (i > 0) {
-                    preSE = sb[(3 * (i - 1)) * N + jj]; preSI = sb[(3 * (i - 1) + 1) * N + jj];
-                    preg = sb[(3 * (i - 1) + 2) * N + jj];
-                    noise_load(a, sh_t, sh_j, i - 1, &pren0, &pren1);
-                } else if (b > b0) {
-                    const int il = a.inner_it - 1;
-                    preSE = sbn[(3 * il) * N + jj]; preSI = sbn[(3 * il + 1) * N + jj];
-                    preg = sbn[(3 * il + 2) * N + jj];
-                    noise_load(a, nsh_t, nsh_j, il, &pren0, &pren1);
+                double eE = 0.0, eI = 0.0, muE = 0.0, muI = 0.0, rE = 0.0;
+                if (own) {
+                    const double SE = preSE, SI = preSI, g = preg, n0 = pren0, n1 = pren1;
+                    if (i > 0) {
+                        preSE = sb[(3 * (i - 1)) * N + jj]; preSI = sb[(3 * (i - 1) + 1) * N + jj];
+                        preg = sb[(3 * (i - 1) + 2) * N + jj];
+                        noise_load(a, sh_t, sh_j, i - 1, &pren0, &pren1);
+                    } else if (b > b0) {
+                        const int il = a.inner_it - 1;
+                        preSE = sbn[(3 * il) * N + jj]; preSI = sbn[(3 * il + 1) * N + jj];
+                        preg = sbn[(3 * il + 2) * N + jj];
+                        noise_load(a, nsh_t, nsh_j, il, &pren0, &pren1);
+                    }
+                    const double IE = c.w_E__I_0 + wp * JN * SE + g * G * JN - wIE * SI;
+                    const double II = c.w_I__I_0 + JN * SE - c.w_II * SI;
+                    const double aE = c.a_E * IE - c.b_E, aI = c.a_I * II - c.b_I;
+                    rE = aE / (1 - exp(-c.d_E * aE));
+                    const double rI = aI / (1 - exp(-c.d_I * aI));
+                    const double SEs = SE + n0 * sig * c.sqrt_dt + c.dt_gamma_E * ((1 - SE) * rE) - c.dt_itau_E * SE;
+                    const double SIs = SI + n1 * sig * c.sqrt_dt + c.dt_gamma_I * rI - c.dt_itau_I * SI;
+                    muE = (SEs > 0.0 && SEs < 1.0) ? lSE : 0.0;
+                    muI = (SIs > 0.0 && SIs < 1.0) ? lSI : 0.0;
+                    eE = muE * c.dt_gamma_E * (1 - SE) * dH(c.a_E, c.b_E, c.d_E, IE);
+                    eI = muI * c.dt_gamma_I * dH(c.a_I, c.b_I, c.d_I, II);
+                    gGp += eE * JN * g;
+                    gwp += eE * JN * SE;
+                    gJN += eE * (wp * SE + G * g) + eI * SE;
                 }
-                const double IE = c.w_E__I_0 + wp * JN * SE + g * G * JN - wIE * SI;
-                const double II = c.w_I__I_0 + JN * SE - c.w_II * SI;
-                const double aE = c.a_E * IE - c.b_E, aI = c.a_I * II - c.b_I;
-                const double rE = aE / (1 - exp(-c.d_E * aE)), rI = aI / (1 - exp(-c.d_I * aI));
-                const double SEs = SE + n0 * sig * c.sqrt_dt + c.dt_gamma_E * ((1 - SE) * rE) - c.dt_itau_E * SE;
-                const double SIs = SI + n1 * sig * c.sqrt_dt + c.dt_gamma_I * rI - c.dt_itau_I * SI;
-                const double muE = (SEs > 0.0 && SEs < 1.0) ? lSE : 0.0;
-                const double muI = (SIs > 0.0 && SIs < 1.0) ? lSI : 0.0;
-                const double eE = muE * c.dt_gamma_E * (1 - SE) * dH(c.a_E, c.b_E, c.d_E, IE);
-                const double eI = muI * c.dt_gamma_I * dH(c.a_I, c.b_I, c.d_I, II);
-                gGp += eE * JN * g;
-                gwp += eE * JN * SE;
-                gJN += eE * (wp * SE + G * g) + eI * SE;
                 lds_barrier();
                 if (act) ev[j] = eE * G * JN;
                 lds_barrier();
                 double coup = 0.0;
                 if (a.sc_lds) {
 #pragma unroll 8
-                    for (int m = 0; m < N; m++) coup += scs[jj * ldsc + m] * ev[m];
+                    for (int m = k0; m < k1; m++) coup += scs[jj * ldsc + m] * ev[m];
                 } else {
 #pragma unroll 8
-                    for (int m = 0; m < N; m++) coup += a.SC[(long long)jj * N + m] * ev[m];
+                    for (int m = k0; m < k1; m++) coup += a.SC[(long long)jj * N + m] * ev[m];
                 }
-                lSE = muE * (1.0 - c.dt_gamma_E * rE - c.dt_itau_E) + eE * wp * JN + eI * JN + coup;
-                lSI = muI * (1.0 - c.dt_itau_I) - eE * wIE - eI * c.w_II;
+                if (ks > 1) {
+                    if (!own) redk[(part - 1) * np + j] = coup;
+                    lds_barrier();
+                    if (own)
+                        for (int p = 1; p < ks; p++) coup += redk[(p - 1) * np + j];
+                }
+                if (own) {
+                    lSE = muE * (1.0 - c.dt_gamma_E * rE - c.dt_itau_E) + eE * wp * JN + eI * JN + coup;
+                    lSI = muI * (1.0 - c.dt_itau_I) - eE * wIE - eI * c.w_II;
+                }
             }
         }
     }
@@ -329,6 +371,10 @@ int fill_args(AdjArgs& A, RWW::C& c, BWConsts& bw, int n_sims, int nodes, int ti
     A.ck = ck; A.n_ck = (A.bw_it + ck - 1) / ck;
     A.SC = SC; A.G = G; A.wp = wp; A.JN = JN; A.wIE = wIE; A.sigma = sigma;
     A.noise = noise; A.sh_nodes = sh_nodes; A.sh_ts = sh_ts;
+    // k partitions: two when a second partition fits the 256-lane launch
+    // bound and there are enough sources to split
+    A.np = ((nodes + 63) / 64) * 64;
+    A.ks = (A.np <= 128 && nodes >= 50) ? 2 : 1;
     RWW::init_constants(c, dt);
     bw = current_bw();
     bw.dt = bw_dt_s; bw.dt_itau = bw.dt * bw.itau;
@@ -366,10 +412,10 @@ int cubnm_rww_grad_forward(int n_sims, int nodes, int time_steps, int BOLD_TR, d
                   d_wIE, d_sigma, d_noise, d_sh_nodes, d_sh_ts) != 0)
         return -1;
     A.bold = d_bold; A.ckpt = d_ckpt;
-    const size_t base = sizeof(double) * (size_t)nodes;
+    const size_t base = sizeof(double) * ((size_t)nodes + (size_t)(A.ks - 1) * A.np);
     const size_t with_sc = base + sizeof(double) * (size_t)nodes * (nodes + 1);
     A.sc_lds = with_sc <= 150 * 1024;
-    const int threads = ((nodes + 63) / 64) * 64;
+    const int threads = A.ks * A.np;
     if (threads <= 256)
         hipLaunchKernelGGL(rww_fwd_kernel<256>, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
     else
@@ -388,10 +434,10 @@ int cubnm_rww_grad_backward(int n_sims, int nodes, int time_steps, int BOLD_TR, 
         return -1;
     A.ckpt = const_cast<double*>(d_ckpt); A.dbold = d_dbold; A.scratch = d_scratch;
     A.gG = d_gG; A.gwp = d_gwp; A.gJN = d_gJN;
-    const size_t base = sizeof(double) * (2 * (size_t)nodes + 16);
+    const size_t base = sizeof(double) * (2 * (size_t)nodes + 16 + (size_t)(A.ks - 1) * A.np);
     const size_t with_sc = base + sizeof(double) * (size_t)nodes * (nodes + 1);
     A.sc_lds = with_sc <= 150 * 1024;
-    const int threads = ((nodes + 63) / 64) * 64;
+    const int threads = A.ks * A.np;
     if (threads <= 256)
         hipLaunchKernelGGL(rww_adj_kernel<256>, dim3(n_sims), dim3(threads), A.sc_lds ? with_sc : base, 0, A, c, bw);
     else

@@ -11,6 +11,8 @@
   gradient entry of each kind.
 * Through torch autograd, the gradient of 1 - corr(FC_sim, FC_emp) with
   respect to G matches central differences of that loss.
+* The same two checks at 60 nodes, where the kernels split the coupling sum
+  (and its transpose) over two partitions of source nodes.
 """
 import numpy as np
 import pytest
@@ -22,29 +24,31 @@ pytestmark = pytest.mark.gpu
 N, S, DUR, TR = 6, 3, 4.0, 0.5
 
 
-def _problem(ck=3):
+def _problem(ck=3, n=N):
     import torch  # noqa: F401
     from cubnm_amd import grad
-    wIE = np.full((S, N), 1.2)
-    sig = np.full((S, N), 0.01)
-    return grad.RwwProblem(cases.synthetic_sc(N, seed=3), wIE, sig, DUR, TR, seed=4, noise_segment_s=1,
+    wIE = np.full((S, n), 1.2)
+    sig = np.full((S, n), 0.01)
+    return grad.RwwProblem(cases.synthetic_sc(n, seed=3), wIE, sig, DUR, TR, seed=4, noise_segment_s=1,
                            checkpoint_bw_steps=ck)
 
 
-def _params(torch):
+def _params(torch, n=N):
     rng = np.random.default_rng(0)
     G = torch.tensor([0.4, 0.9, 1.3], dtype=torch.float64, device="cuda")
-    wp = torch.tensor(rng.uniform(0.8, 1.6, (S, N)), dtype=torch.float64, device="cuda")
-    JN = torch.tensor(rng.uniform(0.1, 0.3, (S, N)), dtype=torch.float64, device="cuda")
+    wp = torch.tensor(rng.uniform(0.8, 1.6, (S, n)), dtype=torch.float64, device="cuda")
+    JN = torch.tensor(rng.uniform(0.1, 0.3, (S, n)), dtype=torch.float64, device="cuda")
     return G, wp, JN
 
 
-def test_forward_matches_run_simulations():
+@pytest.mark.parametrize("n", [N, 60])
+def test_forward_matches_run_simulations(n):
     import torch
     import cubnm_amd  # noqa: F401
     from cubnm_amd import _core
-    prob = _problem()
-    G, wp, JN = _params(torch)
+    N = n
+    prob = _problem(n=n)
+    G, wp, JN = _params(torch, n)
     bold = prob.forward(G, wp, JN).cpu().numpy()
     gp, rp = cases.params("rWW", S, N, G=G.cpu().numpy(), w_p=wp.cpu().numpy(), J_N=JN.cpu().numpy(),
                           wIE=1.2, sigma=0.01, do_fic=False)
@@ -59,10 +63,12 @@ def test_forward_matches_run_simulations():
     assert cases.bold_rel_err(bold.reshape(S, -1), out["sim_bold"]).max() <= 1e-12
 
 
-def test_adjoint_matches_finite_differences():
+@pytest.mark.parametrize("n", [N, 60])
+def test_adjoint_matches_finite_differences(n):
     import torch
-    prob = _problem(ck=3)
-    G, wp, JN = _params(torch)
+    N = n
+    prob = _problem(ck=3, n=n)
+    G, wp, JN = _params(torch, n)
     W = torch.tensor(np.random.default_rng(1).standard_normal((S, prob.bold_len, N)), dtype=torch.float64,
                      device="cuda")
     L = lambda g, w, j: float((prob.forward(g, w, j) * W).sum())
