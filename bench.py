@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--c4-gens", type=int, default=60, help="CMA-ES generations (c4)")
     ap.add_argument("--c4-steps", type=int, default=40, help="maximum Adam steps (c4)")
     ap.add_argument("--c4-duration", type=float, default=120.0, help="simulated seconds (c4)")
+    ap.add_argument("--c4-lr", type=float, default=0.1,
+                    help="Adam learning rate in the unit cube (c4; 0.1 was the best of 0.02, 0.05, 0.1, 0.2, 0.3)")
     ap.add_argument("--c5-sims", type=int, default=8192, help="simulations (c5)")
     ap.add_argument("--c5-check-sims", type=int, default=4,
                     help="simulations whose FC/FCD are recomputed in FP64 on the host (c5)")
@@ -382,7 +384,7 @@ def run_c4(args):
     torch.cuda.empty_cache()
     fit_gd = fit.RegionalFit(mk(256), emp, bounds, 42, 7, nvr)
     log("c4: Adam")
-    h_gd = fit.adam_fit(fit_gd, args.c4_steps, lr=0.02, seed=2, target=best, log=log)
+    h_gd = fit.adam_fit(fit_gd, args.c4_steps, lr=args.c4_lr, seed=2, target=best, log=log)
     reached = [t for t, b in h_gd if b <= best]
     t_gd = reached[0] if reached else None
     line = {
@@ -391,7 +393,8 @@ def run_c4(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded lognormal SC, seeded AR(1) empirical BOLD 100x166 at TR 0.72 s, host-generated noise)",
         "config": {"workload": f"rWW regional fit, 201 parameters, N={N}, {dur:g} s, TR 0.72 s; CMA-ES popsize 64 "
-                               f"x {args.c4_gens} generations vs Adam on 256 starts with adjoint gradients",
+                               f"x {args.c4_gens} generations vs Adam (lr {args.c4_lr:g}) on 256 starts with adjoint "
+                               f"gradients",
                    "cmaes_best_loss": round(best, 6), "cmaes_seconds_to_best": round(t_cma, 2),
                    "grad_seconds_to_cmaes_best": round(t_gd, 2) if t_gd else None,
                    "grad_best_loss": round(h_gd[-1][1], 6), "grad_steps": len(h_gd)},

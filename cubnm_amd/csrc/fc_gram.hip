@@ -31,7 +31,9 @@
 //     by the tests.
 //
 // Tiles: 64 x 64 outputs per workgroup of 4 waves (each 32 x 32 = 2 x 2
-// MFMA blocks); only tiles on or below the diagonal are launched.
+// MFMA blocks); only tiles on or below the diagonal are launched.  The
+// split-BF16 FCD product (long K, few tiles) is further split over K slices
+// whose partial sums are added in slice order.
 #include "engine.hpp"
 #include <algorithm>
 #include <cmath>
@@ -45,6 +47,7 @@ constexpr int KC64 = 16;      // K chunk staged per step, FP64 form
 constexpr int KCB = 32;       // K chunk staged per step, split-BF16 form
 constexpr int FLUSH = 16;     // BF16 chunks accumulated in FP32 before an FP64 flush
 constexpr int ZT = 256;       // threads of the z-score / centring kernels
+constexpr int MAX_SPLIT = 16; // K slices of the split-BF16 FCD product
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -195,17 +198,28 @@ __global__ void __launch_bounds__(256) gram_f64_kernel(const double* X, long lon
 }
 
 // ---------------------------------------------------------------- 2/4. Gram, split-BF16
+// Grid: (matrix, tile, K slice), the slice innermost.  With n_split == 1 the
+// tile's pairs are written to out; otherwise each slice writes its partial
+// sums to part (slice stride ps, matrix stride os, every pair of every
+// matrix written by exactly one tile per slice) and reduce_slices adds the
+// slices in order.  The next K chunk is loaded into registers while the
+// current one is multiplied; the barriers order LDS only, so those loads
+// stay in flight across them.
 __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const unsigned short* Xh, const unsigned short* Xl,
                                                           long long xs, int rows, int K, double* out, long long os,
-                                                          const int* pair_of, int n_tiles) {
+                                                          const int* pair_of, int n_tiles, int n_split,
+                                                          int k_per_split, double* part, long long ps) {
     // rows padded to 40 halfwords (80 B): 16-byte aligned 8-element reads
     __shared__ __attribute__((aligned(16))) unsigned short Ah[GT][KCB + 8], Al[GT][KCB + 8];
     __shared__ __attribute__((aligned(16))) unsigned short Bh[GT][KCB + 8], Bl[GT][KCB + 8];
-    const long long mat = blockIdx.x / n_tiles;
+    const int split = (int)(blockIdx.x % n_split);
+    const long long rest = blockIdx.x / n_split;
+    const long long mat = rest / n_tiles;
     int ti, tj;
-    tri_tile((int)(blockIdx.x % n_tiles), &ti, &tj);
+    tri_tile((int)(rest % n_tiles), &ti, &tj);
     const long long mo = mat * xs;
     const int i0 = ti * GT, j0 = tj * GT;
+    const int kb = split * k_per_split, ke = min(K, kb + k_per_split);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
@@ -213,21 +227,30 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const unsigned short* 
     doublex4 d[2][2];
     for (int a = 0; a < 2; a++)
         for (int c = 0; c < 2; c++) { f[a][c] = floatx4{0.f, 0.f, 0.f, 0.f}; d[a][c] = doublex4{0.0, 0.0, 0.0, 0.0}; }
+    // 64 rows x 32 halfwords = 256 loads of 8 halfwords (16 B) per array
+    const int r = threadIdx.x >> 2, c8 = (threadIdx.x & 3) * 8;
+    const bool ra = i0 + r < rows, rb = j0 + r < rows;
+    const uint4 z = {0u, 0u, 0u, 0u};
+    const long long ga0 = mo + (long long)(i0 + r) * K + c8;
+    const long long gb0 = mo + (long long)(j0 + r) * K + c8;
+    uint4 pah = z, pal = z, pbh = z, pbl = z;
+    if (kb < ke) {
+        if (ra) { pah = *(const uint4*)(Xh + ga0 + kb); pal = *(const uint4*)(Xl + ga0 + kb); }
+        if (rb) { pbh = *(const uint4*)(Xh + gb0 + kb); pbl = *(const uint4*)(Xl + gb0 + kb); }
+    }
     int chunk = 0;
-    for (int k0 = 0; k0 < K; k0 += KCB, chunk++) {
-        __syncthreads();
-        // 64 rows x 32 halfwords = 256 loads of 8 halfwords (16 B) per array
-        {
-            const int r = threadIdx.x >> 2, c8 = (threadIdx.x & 3) * 8;
-            const uint4 z = {0u, 0u, 0u, 0u};
-            const long long ga = mo + (long long)(i0 + r) * K + k0 + c8;
-            const long long gb = mo + (long long)(j0 + r) * K + k0 + c8;
-            *(uint4*)&Ah[r][c8] = i0 + r < rows ? *(const uint4*)(Xh + ga) : z;
-            *(uint4*)&Al[r][c8] = i0 + r < rows ? *(const uint4*)(Xl + ga) : z;
-            *(uint4*)&Bh[r][c8] = j0 + r < rows ? *(const uint4*)(Xh + gb) : z;
-            *(uint4*)&Bl[r][c8] = j0 + r < rows ? *(const uint4*)(Xl + gb) : z;
+    for (int k0 = kb; k0 < ke; k0 += KCB, chunk++) {
+        lds_barrier();  // the previous chunk's LDS reads are done
+        *(uint4*)&Ah[r][c8] = pah;
+        *(uint4*)&Al[r][c8] = pal;
+        *(uint4*)&Bh[r][c8] = pbh;
+        *(uint4*)&Bl[r][c8] = pbl;
+        lds_barrier();
+        if (k0 + KCB < ke) {
+            const int kn = k0 + KCB;
+            if (ra) { pah = *(const uint4*)(Xh + ga0 + kn); pal = *(const uint4*)(Xl + ga0 + kn); }
+            if (rb) { pbh = *(const uint4*)(Xh + gb0 + kn); pbl = *(const uint4*)(Xl + gb0 + kn); }
         }
-        __syncthreads();
         bf16x8 ah[2], al[2], bh[2], bl[2];
         for (int x = 0; x < 2; x++) {
             // 16x16x32 bf16 operands: lane l holds row (or column) l%16, k = 8*(l/16) .. +7
@@ -245,20 +268,30 @@ __global__ void __launch_bounds__(256) gram_bf16x3_kernel(const unsigned short* 
         if ((chunk + 1) % FLUSH == 0) {
             for (int x = 0; x < 2; x++)
                 for (int y = 0; y < 2; y++)
-                    for (int r = 0; r < 4; r++) { d[x][y][r] += (double)f[x][y][r]; f[x][y][r] = 0.f; }
+                    for (int q = 0; q < 4; q++) { d[x][y][q] += (double)f[x][y][q]; f[x][y][q] = 0.f; }
         }
     }
-    double* o = out + mat * os;
+    double* o = part ? part + split * ps + mat * os : out + mat * os;
     for (int x = 0; x < 2; x++)
         for (int y = 0; y < 2; y++)
-            for (int r = 0; r < 4; r++) {
-                // f32 16x16 D layout: register r of lane l holds row 4*(l/16) + r, column l%16
-                const int i = i0 + wr + 16 * x + 4 * lk + r, j = j0 + wc + 16 * y + li;
+            for (int q = 0; q < 4; q++) {
+                // f32 16x16 D layout: register q of lane l holds row 4*(l/16) + q, column l%16
+                const int i = i0 + wr + 16 * x + 4 * lk + q, j = j0 + wc + 16 * y + li;
                 if (i < rows && j < i) {
                     const long long p = pair_index(i, j, pair_of, rows);
-                    if (p >= 0) o[p] = d[x][y][r] + (double)f[x][y][r];
+                    if (p >= 0) o[p] = d[x][y][q] + (double)f[x][y][q];
                 }
             }
+}
+
+// out[i] = sum over slices s (in order) of part[s * ps + i], i < n.
+__global__ void __launch_bounds__(256) reduce_slices(const double* part, long long ps, int n_split, long long n,
+                                                     double* out) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        double v = part[i];
+        for (int q = 1; q < n_split; q++) v += part[q * ps + i];
+        out[i] = v;
+    }
 }
 
 // ---------------------------------------------------------------- host pipeline
@@ -281,6 +314,7 @@ GramLayout gram_layout(const FcArgs& a, int max_win_len, bool bf16) {
     size_t per = e * (size_t)a.nodes * g.k_fc + 8 * (size_t)a.n_windows * a.nodes * g.k_w;
     per += 8 * (size_t)a.n_windows * g.p_ld;                 // FP64 window-FC vectors
     if (bf16) per += 4 * (size_t)a.n_windows * g.p_ld;       // their hi/lo split
+    if (bf16) per += 8 * (size_t)MAX_SPLIT * a.n_window_pairs;  // FCD K-slice partial sums
     g.per_sim_bytes = per;
     return g;
 }
@@ -316,7 +350,12 @@ hipError_t launch_fc_gram(const FcArgs& a, int max_win_len, int split_bf16, cons
     w += 8 * n_w;
     double* U = (double*)w;
     w += 8 * n_u;
-    if (bf16) { uh = (unsigned short*)w; ul = uh + n_u; }
+    double* part = nullptr;
+    if (bf16) {
+        uh = (unsigned short*)w; ul = uh + n_u;
+        w += 4 * n_u;
+        part = (double*)w;  // [MAX_SPLIT][batch][n_window_pairs]
+    }
     const int mats = 1 + (a.n_window_pairs > 0 ? W : 0);
     if (W > 0 && a.n_window_pairs > 0) {
         hipError_t e0 = hipMemsetAsync(U, 0, 8 * n_u, s);   // pair padding stays 0
@@ -337,7 +376,8 @@ hipError_t launch_fc_gram(const FcArgs& a, int max_win_len, int split_bf16, cons
         const int tn = tri_tiles(N), tw = tri_tiles(W);
         if (bf16)
             hipLaunchKernelGGL(gram_bf16x3_kernel, dim3(tn * nb), dim3(256), 0, s, h_fc, l_fc,
-                               (long long)N * g.k_fc, N, g.k_fc, fc, (long long)a.n_pairs, pair_of, tn);
+                               (long long)N * g.k_fc, N, g.k_fc, fc, (long long)a.n_pairs, pair_of, tn, 1, g.k_fc,
+                               (double*)nullptr, 0LL);
         else
             hipLaunchKernelGGL(gram_f64_kernel, dim3(tn * nb), dim3(256), 0, s, x_fc,
                                (long long)N * g.k_fc, N, g.k_fc, fc, (long long)a.n_pairs, pair_of, tn);
@@ -347,9 +387,19 @@ hipError_t launch_fc_gram(const FcArgs& a, int max_win_len, int split_bf16, cons
             double* fcd = a.fcd + (long long)s0 * a.n_window_pairs;
             if (bf16) {
                 hipLaunchKernelGGL(center_kernel<true>, dim3(nb * W), dim3(ZT), 0, s, U, a.n_pairs, g.p_ld, uh, ul);
-                hipLaunchKernelGGL(gram_bf16x3_kernel, dim3(tw * nb), dim3(256), 0, s, uh, ul,
+                // K slices: about 2048 workgroups (8 per CU) over the batch,
+                // each slice at least 32 chunks long
+                const int chunks = g.p_ld / KCB;
+                int n_split = std::max(1, std::min({MAX_SPLIT, chunks / 32, (2048 + tw * nb - 1) / (tw * nb)}));
+                const int kps = ((chunks + n_split - 1) / n_split) * KCB;
+                n_split = (g.p_ld + kps - 1) / kps;
+                const long long ps = (long long)nb * a.n_window_pairs;
+                hipLaunchKernelGGL(gram_bf16x3_kernel, dim3(tw * nb * n_split), dim3(256), 0, s, uh, ul,
                                    (long long)W * g.p_ld, W, g.p_ld, fcd, (long long)a.n_window_pairs,
-                                   (const int*)nullptr, tw);
+                                   (const int*)nullptr, tw, n_split, kps, n_split > 1 ? part : (double*)nullptr, ps);
+                if (n_split > 1)
+                    hipLaunchKernelGGL(reduce_slices, dim3((int)std::min<long long>((ps + 255) / 256, 4096)),
+                                       dim3(256), 0, s, part, ps, n_split, ps, fcd);
             } else {
                 hipLaunchKernelGGL(center_kernel<false>, dim3(nb * W), dim3(ZT), 0, s, U, a.n_pairs, g.p_ld,
                                    nullptr, nullptr);

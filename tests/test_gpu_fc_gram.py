@@ -81,8 +81,10 @@ def test_fp64_gram_equals_lds_kernel(core):
     np.testing.assert_allclose(b["sim_fcd_trils"], a["sim_fcd_trils"], rtol=0, atol=1e-13)
 
 
-@pytest.mark.parametrize("nodes", [100, 400])
+@pytest.mark.parametrize("nodes", [24, 100, 400])
 def test_split_bf16_gram_error(core, nodes):
+    # the FCD product runs in one K slice at 24 nodes (276 pairs), in 4 K
+    # slices at 100 and in 16 at 400 (partial sums added in slice order)
     duration, TR = (40, 0.72) if nodes == 400 else (40, 1.0)
     out, path, d = _run(core, nodes, 2, duration, TR, 10, 1, 0, False, 2)
     assert path == 2
