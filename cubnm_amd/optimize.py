@@ -1,0 +1,355 @@
+"""Optimisation problems over a SimGroup's parameters, and the grid and
+CMA-ES drivers that evaluate them (the caller side of the simulation core).
+
+``BNMProblem`` follows the reference's ``cubnm.optimize.BNMProblem``
+(optimize.py:40-619) without its pymoo base class: the same constructor
+arguments and attributes (``free_params``, ``fixed_params``, ``lb``, ``ub``,
+``ndim``, ``obj_names``, ``n_obj``, ``xl``, ``xu``), the same mapping from
+normalised candidates X in [0, 1]^ndim to simulation parameters
+(``_get_Xt``, ``_get_X``, ``_get_sim_params``: global parameters,
+regional biases, node-group heterogeneity and map-based heterogeneity with
+the optional range normalisation), the same ``eval`` (run + score) and the
+same ``_evaluate(X, out, scores=[...])`` calling convention, including the
+rWW FIC penalty hook.  ``GridOptimizer`` (optimize.py:1168-1233) and
+``CMAESOptimizer`` (optimize.py:1386-1486; here driven by this package's
+own CMA-ES, ``fit.CMAES``, in the unit cube) produce ``history``,
+``opt_history`` and ``opt`` tables with the reference's columns.
+"""
+import itertools
+import logging
+import os
+
+import numpy as np
+import pandas as pd
+
+from . import sim, utils
+
+logger = logging.getLogger(__name__)
+
+
+class BNMProblem:
+    def __init__(self, model, params, emp_fc_tril=None, emp_fcd_tril=None, emp_bold=None, het_params=(),
+                 het_params_range="same", maps=None, maps_coef_range="auto", node_grouping=None, multiobj=False,
+                 **kwargs):
+        """Free parameters are the entries of ``params`` given as (min, max)
+        tuples, fixed ones are given as values.  ``het_params`` (regional
+        parameters) vary across nodes through ``node_grouping`` ('node',
+        'sym', a path or an array of group labels: one free parameter per
+        group, named f"{param}{group}") or through ``maps`` (n_maps x nodes:
+        one free coefficient per map, f"{param}scale{map_idx}", the regional
+        value being bias x (1 + sum_m coef_m map_m), optionally renormalised
+        into ``het_params_range``).  Other keyword arguments construct the
+        model's SimGroup."""
+        self.model = model
+        self.params = params
+        self.emp_fc_tril = emp_fc_tril
+        self.emp_fcd_tril = emp_fcd_tril
+        self.emp_bold = emp_bold
+        self.het_params = list(het_params)
+        self.input_maps = maps
+        self.maps_coef_range = maps_coef_range
+        self.het_params_range = het_params_range
+        self.node_grouping = node_grouping
+        self.multiobj = multiobj
+        self.sim_group = getattr(sim, f"{model}SimGroup")(**kwargs)
+        sg = self.sim_group
+        # empirical targets (optimize.py:143-187)
+        if emp_bold is not None:
+            if emp_fc_tril is not None or emp_fcd_tril is not None:
+                logger.warning("Both empirical BOLD and empirical FC/FCD are provided; "
+                               "FC/FCD are computed from the BOLD")
+            if sg.do_fc:
+                self.emp_fc_tril = utils.calculate_fc(emp_bold, sg.exc_interhemispheric, return_tril=True)
+            if sg.do_fcd:
+                self.emp_fcd_tril = utils.calculate_fcd(emp_bold, sg.window_size_TRs, sg.window_step_TRs,
+                                                        drop_edges=sg.fcd_drop_edges,
+                                                        exc_interhemispheric=sg.exc_interhemispheric)
+        else:
+            # flags from the targets given, in the reference's order (the
+            # SimGroup setters reject FCD without FC and GOF terms without
+            # their measure)
+            if self.emp_fc_tril is not None:
+                sg.do_fc = True
+                if self.emp_fcd_tril is not None:
+                    sg.do_fcd = True
+            else:
+                sg.do_fc = False
+            if self.emp_fcd_tril is not None:
+                # the core computes FC and FCD together
+                sg.do_fc = True
+                sg.do_fcd = True
+            else:
+                sg.do_fcd = False
+        if node_grouping is not None and maps is not None:
+            raise ValueError("Both `node_grouping` and `maps` cannot be used")
+        self.global_params = list(sg.global_param_names) + ["v"]
+        self.regional_params = list(sg.regional_param_names)
+        self.free_params, lb, ub = [], [], []
+        # node groups (optimize.py:205-236)
+        self.is_node_based = node_grouping is not None
+        if self.is_node_based:
+            if isinstance(node_grouping, (str, os.PathLike)) and node_grouping == "node":
+                self.memberships = np.arange(sg.nodes)
+            elif isinstance(node_grouping, (str, os.PathLike)) and node_grouping == "sym":
+                if sg.nodes % 2:
+                    raise ValueError("Number of nodes must be even")
+                self.memberships = np.tile(np.arange(sg.nodes // 2), 2)
+            elif isinstance(node_grouping, (str, os.PathLike)):
+                self.memberships = np.loadtxt(node_grouping).astype(int)
+            elif isinstance(node_grouping, np.ndarray):
+                self.memberships = node_grouping.astype(int)
+            else:
+                raise ValueError("Invalid node_grouping provided.")
+            if self.memberships.shape != (sg.nodes,):
+                raise ValueError(f"Invalid node_grouping shape: {self.memberships.shape}.")
+            self.node_groups = np.unique(self.memberships)
+        # global, bias and group parameters (optimize.py:238-261)
+        for p, v in params.items():
+            if p not in self.global_params + self.regional_params or not isinstance(v, tuple):
+                continue
+            if self.is_node_based and p in self.regional_params and p in self.het_params:
+                for group in self.node_groups:
+                    name = f"{p}{group}"
+                    if name not in params:
+                        self.free_params.append(name)
+                        lb.append(v[0])
+                        ub.append(v[1])
+            else:
+                self.free_params.append(p)
+                lb.append(v[0])
+                ub.append(v[1])
+        self.fixed_params = [p for p in params if p not in self.free_params]
+        # map coefficients (optimize.py:263-320)
+        self.is_map_based = maps is not None
+        if self.is_map_based:
+            self.maps = np.loadtxt(maps) if isinstance(maps, (str, os.PathLike)) else np.asarray(maps, float)
+            if self.maps.ndim != 2 or self.maps.shape[1] != sg.nodes:
+                raise ValueError(f"Maps second dimension {self.maps.shape[-1]} != nodes {sg.nodes}")
+            for p in self.het_params:
+                for m in range(self.maps.shape[0]):
+                    name = f"{p}scale{m}"
+                    if name in params:
+                        continue
+                    if maps_coef_range == "auto":
+                        mx, mn = self.maps[m].max(), self.maps[m].min()
+                        if mn < 0 < mx:
+                            # e.g. z-scored: the map can scale the bias down to 0 at either end
+                            lo = np.ceil((-1 / mx) / 0.1) * 0.1
+                            hi = np.floor((-1 / mn) / 0.1) * 0.1
+                        else:
+                            lo, hi = 0.0, 1.0
+                    elif isinstance(maps_coef_range, tuple):
+                        lo, hi = maps_coef_range
+                    elif isinstance(maps_coef_range, list):
+                        lo, hi = maps_coef_range[m]
+                    else:
+                        raise ValueError("Invalid maps_coef_range provided")
+                    self.free_params.append(name)
+                    lb.append(lo)
+                    ub.append(hi)
+            if het_params_range == "same":
+                self._het_params_range = {}
+                for p in self.het_params:
+                    if p not in params:
+                        raise ValueError(f"Parameter {p} is not defined in params")
+                    if not isinstance(params[p], tuple):
+                        raise ValueError(f"Parameter {p} baseline cannot be fixed when map-based heterogeneity "
+                                         "is applied and het_params_range is set to 'same'")
+                    self._het_params_range[p] = params[p]
+            elif isinstance(het_params_range, dict) or het_params_range is None:
+                self._het_params_range = het_params_range
+            else:
+                raise ValueError("Invalid het_params_range provided")
+        self.lb = np.array(lb, dtype=float)
+        self.ub = np.array(ub, dtype=float)
+        self.ndim = len(self.free_params)
+        if multiobj:
+            # costs are minimised: +term becomes -term and vice versa
+            self.obj_names = [("+" + t[1:]) if t.startswith("-") else ("-" + t[1:]) for t in sg.gof_terms]
+        else:
+            self.obj_names = ["cost"]
+        self.n_obj = len(self.obj_names)
+        sg._problem_init(self)
+        self.n_var = self.ndim
+        self.xl = np.zeros(self.ndim)
+        self.xu = np.ones(self.ndim)
+
+    def get_config(self, include_sim_group=True, include_N=False):
+        config = {
+            "model": self.model, "params": self.params, "het_params": self.het_params, "maps": self.input_maps,
+            "node_grouping": self.node_grouping, "maps_coef_range": self.maps_coef_range,
+            "het_params_range": self.het_params_range, "emp_fc_tril": self.emp_fc_tril,
+            "emp_fcd_tril": self.emp_fcd_tril, "emp_bold": self.emp_bold,
+        }
+        if include_N:
+            config["N"] = self.sim_group.N
+        if include_sim_group:
+            config.update(self.sim_group.get_config())
+        return config
+
+    def _get_Xt(self, X):
+        """Normalised candidates (N, ndim) in [0, 1] -> parameter values."""
+        return np.asarray(X, float) * (self.ub - self.lb) + self.lb
+
+    def _get_X(self, Xt):
+        """Parameter values -> normalised candidates."""
+        return (np.asarray(Xt, float) - self.lb) / (self.ub - self.lb)
+
+    def _get_sim_params(self, X):
+        """Global (N,) and regional (N, nodes) parameter arrays of the
+        SimGroup for candidates X (optimize.py:432-537)."""
+        X = np.atleast_2d(np.asarray(X, float))
+        N, nodes = X.shape[0], self.sim_group.nodes
+        Xt = self._get_Xt(X)
+        col = {p: i for i, p in enumerate(self.free_params)}
+        out = {}
+        for p in self.fixed_params:
+            if isinstance(self.params[p], tuple):
+                continue  # a group-heterogeneous parameter's range: set per group below
+            if p in self.global_params:
+                out[p] = np.repeat(float(self.params[p]), N)
+            elif p in self.regional_params:
+                out[p] = np.broadcast_to(np.asarray(self.params[p], float), (N, nodes)).copy()
+        for p in self.free_params:
+            if p in self.global_params:
+                out[p] = Xt[:, col[p]].copy()
+            elif p in self.regional_params:
+                out[p] = np.repeat(Xt[:, col[p]][:, None], nodes, axis=1)
+        if self.is_map_based:
+            for p in self.het_params:
+                for s in range(N):
+                    scal = np.ones(nodes)
+                    for m in range(self.maps.shape[0]):
+                        name = f"{p}scale{m}"
+                        coef = self.params[name] if name in self.fixed_params else Xt[s, col[name]]
+                        scal = scal + self.maps[m] * coef
+                    out[p][s] *= scal
+                    if self._het_params_range is not None:
+                        # min-max renormalise into the range when an end is
+                        # outside it; an end that is inside stays where it is
+                        lo_lim, hi_lim = self._het_params_range[p]
+                        umin, umax = out[p][s].min(), out[p][s].max()
+                        tmin = min(max(lo_lim, umin), hi_lim)
+                        tmax = max(min(hi_lim, umax), lo_lim)
+                        if umin < tmin or umax > tmax:
+                            out[p][s] = (out[p][s] - umin) / (umax - umin) * (tmax - tmin) + tmin
+        if self.is_node_based:
+            for p in self.het_params:
+                v = np.zeros((N, nodes))
+                for group in self.node_groups:
+                    name = f"{p}{group}"
+                    sel = self.memberships == group
+                    if name in self.fixed_params:
+                        v[:, sel] = self.params[name]
+                    else:
+                        v[:, sel] = Xt[:, col[name]][:, None]
+                out[p] = v
+        return out
+
+    def _set_sim_params(self, X):
+        self.sim_group.param_lists.update(self._get_sim_params(X))
+        self.sim_group._set_default_params(missing=True)
+
+    def eval(self, X, skip_run=False):
+        """Run the candidates' simulations and score them (pandas DataFrame,
+        one row per candidate)."""
+        self.sim_group.N = np.atleast_2d(X).shape[0]
+        self._set_sim_params(X)
+        if not skip_run:
+            self.sim_group.run()
+        return self.sim_group.score(self.emp_fc_tril, self.emp_fcd_tril)
+
+    def _evaluate(self, X, out, *args, **kwargs):
+        """pymoo calling convention: out['F'] = costs (N, n_obj); the scores
+        table is appended to kwargs['scores']."""
+        skip_run = kwargs.pop("skip_run", False)
+        scores = self.eval(X, skip_run=skip_run)
+        kwargs.setdefault("scores", []).append(scores)
+        if self.multiobj:
+            out["F"] = -scores.loc[:, self.sim_group.gof_terms].values
+        else:
+            out["F"] = -scores.loc[:, "+gof"].values
+        self.sim_group._problem_evaluate(self, X, out, *args, **kwargs)
+
+    def evaluate(self, X):
+        """Costs (N,) or (N, n_obj) and the scores table of candidates X."""
+        out, scores = {}, []
+        self._evaluate(X, out, scores=scores)
+        return out["F"], scores[0]
+
+
+class GridOptimizer:
+    """Every point of a grid over the free parameters, evaluated in one
+    batch (optimize.py:1168-1233)."""
+
+    def optimize(self, problem, grid_shape):
+        self.problem = problem
+        if isinstance(grid_shape, int):
+            grid_shape = {p: grid_shape for p in problem.free_params}
+        self.grid_shape = grid_shape
+        axes = [np.linspace(0, 1, grid_shape[p]) for p in problem.free_params]
+        X = np.array(list(itertools.product(*axes)))
+        self.popsize = X.shape[0]
+        out, scores = {}, []
+        problem._evaluate(X, out, scores=scores)
+        scores = scores[0]
+        scores["cost"] = out["F"]
+        Xt = pd.DataFrame(problem._get_Xt(X), columns=problem.free_params)
+        self.history = pd.concat([Xt, scores], axis=1)
+        self.opt = self.history.loc[self.history["cost"].idxmin()]
+        self.is_fit = True
+        return self.opt
+
+
+class CMAESOptimizer:
+    """CMA-ES in the unit cube over a BNMProblem's free parameters
+    (optimize.py:1386-1486): with x0=None the first generation is popsize
+    uniform samples whose best point becomes the initial mean; sigma 0.5;
+    popsize candidates per generation for n_iter generations after it.
+    Uses this package's CMA-ES (fit.CMAES, Hansen's tutorial defaults,
+    samples clipped to the cube) instead of pymoo's wrapper of `cma`, so the
+    trajectories are not the reference's."""
+
+    def __init__(self, popsize, n_iter=2, seed=0, sigma=0.5, x0=None, print_history=False):
+        self.popsize, self.n_iter, self.seed, self.sigma, self.x0 = popsize, n_iter, seed, sigma, x0
+        self.print_history = print_history
+
+    def setup_problem(self, problem):
+        if problem.n_obj != 1:
+            raise ValueError("CMA-ES optimises a single objective")
+        self.problem = problem
+
+    def _record(self, X, F, scores, gen):
+        Xt = pd.DataFrame(self.problem._get_Xt(X), columns=self.problem.free_params)
+        Xt.index.name = "sim_idx"
+        res = pd.concat([Xt, pd.DataFrame({"cost": F}), scores], axis=1)
+        res["gen"] = gen
+        if self.print_history:
+            print(res.to_string())
+        self.history.append(res)
+        self.opt_history.append(res.loc[res["cost"].argmin()])
+
+    def optimize(self):
+        from .fit import CMAES
+        rng = np.random.default_rng(self.seed)
+        self.history, self.opt_history = [], []
+        if self.x0 is None:
+            X = rng.uniform(0, 1, (self.popsize, self.problem.ndim))
+            F, scores = self.problem.evaluate(X)
+            F = np.where(np.isnan(F), np.inf, F)
+            self._record(X, F, scores, 0)
+            x0 = X[int(np.argmin(F))]
+        else:
+            x0 = self.problem._get_X(np.asarray(self.x0, float))
+        es = CMAES(x0, self.sigma, self.popsize, seed=self.seed + 1)
+        for g in range(self.n_iter):
+            X = es.ask()
+            F, scores = self.problem.evaluate(X)
+            F = np.where(np.isnan(F), np.inf, F)
+            es.tell(X, F)
+            self._record(X, F, scores, g + 1)
+        self.history = pd.concat(self.history, axis=0).reset_index(drop=False)
+        self.opt_history = pd.DataFrame(self.opt_history).reset_index(drop=True)
+        self.opt = self.history.loc[self.history["cost"].argmin()]
+        self.is_fit = True
+        return self.opt

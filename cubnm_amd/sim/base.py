@@ -456,6 +456,13 @@ class SimGroup:
         gc.collect()
 
     # ---- goodness of fit ------------------------------------------------------
+    # ---- hooks of optimize.BNMProblem (model-specific costs) --------------
+    def _problem_init(self, problem):
+        """Model-specific checks and objectives of a BNMProblem (none here)."""
+
+    def _problem_evaluate(self, problem, X, out, *args, **kwargs):
+        """Model-specific additions to a BNMProblem's costs (none here)."""
+
     def score(self, emp_fc_tril=None, emp_fcd_tril=None, emp_bold=None, force_cpu=False, usable_mem=None):
         """GOF terms per simulation (pandas DataFrame, one row per
         simulation): +fc_corr (Pearson r), -fc_diff (-|mean difference|),

@@ -71,6 +71,26 @@ class rWWSimGroup(SimGroup):
             self.fic_ntrials = self._global_ints[0]
             self.param_lists["wIE"] = self._regional_params[2].reshape(self.N, -1)
 
+    def _problem_init(self, problem):
+        """wIE cannot be heterogeneous when FIC sets it; with several
+        objectives the FIC penalty is one of them (rww.py:254-271)."""
+        if self.do_fic and "wIE" in problem.het_params:
+            raise ValueError("In rWW wIE should not be specified as a heterogeneous parameter when FIC is done")
+        if problem.multiobj and self.do_fic and self.fic_penalty_scale > 0:
+            problem.obj_names.append("-fic_penalty")
+            problem.n_obj += 1
+
+    def _problem_evaluate(self, problem, X, out, *args, **kwargs):
+        """The FIC penalty enters the cost (single objective: subtracted,
+        being a negative score) or is appended as an objective
+        (rww.py:273-297)."""
+        scores = kwargs["scores"][-1]
+        if self.do_fic and self.fic_penalty_scale > 0:
+            if problem.multiobj:
+                out["F"] = np.concatenate([out["F"], -scores.loc[:, ["-fic_penalty"]].values], axis=1)
+            else:
+                out["F"] = out["F"] - scores.loc[:, "-fic_penalty"].values
+
     def score(self, emp_fc_tril=None, emp_fcd_tril=None, emp_bold=None, **kwargs):
         """GOF terms plus ``-fic_penalty`` (with FIC): per node, a mean r_E
         more than 1 Hz from 3 Hz costs 1 - exp(-0.05 (|dev| - 1)); the sum

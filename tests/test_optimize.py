@@ -1,0 +1,185 @@
+"""Optimisation layer (cubnm_amd.optimize) on the CPU.
+
+The parameter mapping of BNMProblem is host arithmetic, checked against
+what it must satisfy rather than against a restated formula:
+  * the unit cube maps linearly onto the bounds and back (round trip);
+  * fixed parameters are constant, free global and bias parameters follow
+    their column, group parameters are shared exactly by their group's
+    members ('sym': node i and node i + N/2; 'node': one per node; a fixed
+    group value stays fixed);
+  * map coefficients: the automatic range keeps every node's value
+    non-negative at both ends for a z-scored map and is a multiple of 0.1;
+    (0, 1) for a positive map; the regional value is bias x (1 + sum of
+    coefficient x map), which at zero coefficients is the bias;
+  * range normalisation maps the values into the range with an affine,
+    order-preserving map, leaves them unchanged when inside, and keeps an
+    in-range end where it is;
+  * the reference's argument errors.
+With run_simulations swapped for the oracle (test infrastructure), the grid
+and CMA-ES drivers evaluate the problem end to end: costs are -(+gof), the
+FIC penalty enters the cost, and the grid's rows equal a direct SimGroup run.
+"""
+import numpy as np
+import pytest
+
+import cases
+
+N = 6
+
+
+@pytest.fixture
+def oracle_core(monkeypatch):
+    """Simulations on the oracle and the two GPU GOF reductions as SciPy."""
+    import scipy.stats
+    from cubnm_amd import _core, utils
+    monkeypatch.setattr(_core, "run_simulations", lambda *a: cases.run_oracle(a))
+    monkeypatch.setattr(utils, "fc_corr_device",
+                        lambda sim, emp, usable_mem=None: np.array([scipy.stats.pearsonr(x, emp)[0] for x in sim]))
+    monkeypatch.setattr(utils, "fcd_ks_device",
+                        lambda sim, emp, usable_mem=None: np.array([scipy.stats.ks_2samp(x, emp).statistic
+                                                                    for x in sim]))
+
+
+def _problem(params, **kw):
+    from cubnm_amd.optimize import BNMProblem
+    base = dict(model="rWW", params=params, duration=12, TR=1, sc=cases.synthetic_sc(N, seed=1), window_size=4,
+                window_step=2, bold_remove_s=2, do_fic=False)
+    base.update(kw)
+    if "emp_fc_tril" not in kw and "emp_bold" not in kw:
+        rng = np.random.default_rng(0)
+        base["emp_fc_tril"] = np.tanh(rng.standard_normal(N * (N - 1) // 2))
+        base["emp_fcd_tril"] = np.tanh(rng.standard_normal(6))
+    return BNMProblem(**base)
+
+
+def test_homogeneous_mapping_and_round_trip():
+    p = _problem({"G": (0.5, 3.0), "w_p": (0.0, 2.0), "J_N": 0.15})
+    assert p.free_params == ["G", "w_p"] and set(p.fixed_params) == {"J_N"} and p.ndim == 2
+    X = np.array([[0.0, 0.0], [1.0, 1.0], [0.5, 0.25]])
+    np.testing.assert_allclose(p._get_X(p._get_Xt(X)), X, atol=1e-15)
+    sp = p._get_sim_params(X)
+    np.testing.assert_allclose(sp["G"], [0.5, 3.0, 1.75])
+    assert sp["w_p"].shape == (3, N)
+    np.testing.assert_allclose(sp["w_p"], np.repeat([[0.0], [2.0], [0.5]], N, axis=1))
+    np.testing.assert_array_equal(sp["J_N"], np.full((3, N), 0.15))
+    assert p.xl.tolist() == [0, 0] and p.xu.tolist() == [1, 1] and p.n_obj == 1
+
+
+def test_node_groups_sym_node_and_fixed_group():
+    p = _problem({"G": (0.5, 3.0), "w_p": (0.0, 2.0)}, het_params=["w_p"], node_grouping="sym")
+    assert p.free_params == ["G"] + [f"w_p{g}" for g in range(N // 2)]
+    X = np.random.default_rng(1).uniform(0, 1, (4, p.ndim))
+    wp = p._get_sim_params(X)["w_p"]
+    np.testing.assert_array_equal(wp[:, :N // 2], wp[:, N // 2:])
+    np.testing.assert_allclose(wp[:, :N // 2], 2.0 * X[:, 1:])
+    q = _problem({"G": (0.5, 3.0), "w_p": (0.0, 2.0), "w_p3": 0.7}, het_params=["w_p"], node_grouping="node")
+    assert "w_p3" not in q.free_params and len(q.free_params) == 1 + N - 1
+    wq = q._get_sim_params(np.full((2, q.ndim), 0.5))["w_p"]
+    np.testing.assert_array_equal(wq[:, 3], [0.7, 0.7])
+    np.testing.assert_allclose(np.delete(wq, 3, axis=1), 1.0)
+    groups = np.array([0, 0, 1, 1, 1, 2])
+    r = _problem({"G": (0.5, 3.0), "J_N": (0.1, 0.3)}, het_params=["J_N"], node_grouping=groups)
+    jn = r._get_sim_params(np.array([[0.0, 0.0, 0.5, 1.0]]))["J_N"][0]
+    np.testing.assert_allclose(jn, [0.1, 0.1, 0.2, 0.2, 0.2, 0.3])
+
+
+def test_map_coefficients_and_ranges():
+    rng = np.random.default_rng(2)
+    z = rng.standard_normal(N)
+    z = (z - z.mean()) / z.std()
+    pos = rng.uniform(0.1, 1.0, N)
+    maps = np.vstack([z, pos])
+    p = _problem({"G": (0.5, 3.0), "w_p": (0.5, 2.0)}, het_params=["w_p"], maps=maps, het_params_range=None)
+    assert p.free_params == ["G", "w_p", "w_pscale0", "w_pscale1"]
+    lo, hi = p.lb[2], p.ub[2]
+    assert lo < 0 < hi
+    for v in (lo, hi):
+        assert abs(round(v * 10) - v * 10) < 1e-9
+        assert np.all(1 + v * z >= -1e-12)            # the bias is never flipped in sign
+    assert (p.lb[3], p.ub[3]) == (0.0, 1.0)
+    # zero coefficients: every node at the bias
+    Xz = np.array([[0.5, 0.5, p._get_X([0, 0, 0, 0])[2], 0.0]])
+    np.testing.assert_allclose(p._get_sim_params(Xz)["w_p"], np.full((1, N), 1.25))
+    # bias x (1 + c0 z + c1 pos)
+    X = np.array([[0.5, 1.0, 1.0, 1.0]])
+    np.testing.assert_allclose(p._get_sim_params(X)["w_p"][0], 2.0 * (1 + hi * z + 1.0 * pos))
+
+
+def test_range_normalisation():
+    z = np.array([-2.0, -1.0, 0.0, 0.5, 1.0, 1.5])
+    p = _problem({"G": (0.5, 3.0), "w_p": (0.0, 2.0)}, het_params=["w_p"], maps=z[None, :],
+                 maps_coef_range=(-1.0, 1.0))
+    raw = _problem({"G": (0.5, 3.0), "w_p": (0.0, 2.0)}, het_params=["w_p"], maps=z[None, :],
+                   maps_coef_range=(-1.0, 1.0), het_params_range=None)
+    for X in ([0.5, 1.0, 1.0], [0.5, 0.5, 0.6], [0.5, 0.25, 0.5]):
+        X = np.array([X])
+        v, r = p._get_sim_params(X)["w_p"][0], raw._get_sim_params(X)["w_p"][0]
+        assert v.min() >= -1e-12 and v.max() <= 2.0 + 1e-12
+        if r.min() >= 0 and r.max() <= 2:
+            np.testing.assert_allclose(v, r)          # inside the range: unchanged
+        else:
+            np.testing.assert_array_equal(np.argsort(v, kind="stable"), np.argsort(r, kind="stable"))
+            a = np.polyfit(r, v, 1)
+            np.testing.assert_allclose(np.polyval(a, r), v, atol=1e-12)   # affine
+            if 0 <= r.max() <= 2:
+                assert abs(v.max() - r.max()) < 1e-12   # the in-range end stays
+            if 0 <= r.min() <= 2:
+                assert abs(v.min() - r.min()) < 1e-12
+
+
+def test_argument_errors():
+    with pytest.raises(ValueError, match="node_grouping"):
+        _problem({"G": (0.5, 3.0), "w_p": (0, 2)}, het_params=["w_p"], maps=np.ones((1, N)), node_grouping="node")
+    with pytest.raises(ValueError, match="wIE"):
+        _problem({"G": (0.5, 3.0), "wIE": (0.5, 2)}, het_params=["wIE"], node_grouping="node", do_fic=True)
+    with pytest.raises(ValueError, match="cannot be fixed"):
+        _problem({"G": (0.5, 3.0), "w_p": 1.0}, het_params=["w_p"], maps=np.ones((1, N)))
+    with pytest.raises(ValueError, match="shape"):
+        _problem({"G": (0.5, 3.0), "w_p": (0, 2)}, het_params=["w_p"], node_grouping=np.zeros(N + 1))
+
+
+def test_multiobjective_names():
+    p = _problem({"G": (0.5, 3.0)}, multiobj=True, do_fic=True)
+    assert p.obj_names == ["-fc_corr", "+fcd_ks", "-fic_penalty"] and p.n_obj == 3
+
+
+def test_grid_optimizer_end_to_end(oracle_core):
+    from cubnm_amd.optimize import GridOptimizer
+    p = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5), "J_N": 0.15})
+    go = GridOptimizer()
+    opt = go.optimize(p, grid_shape={"G": 2, "w_p": 2})
+    h = go.history
+    assert len(h) == 4 and list(h.columns[:2]) == ["G", "w_p"]
+    np.testing.assert_allclose(h["G"], [0.5, 0.5, 3.0, 3.0])
+    np.testing.assert_allclose(h["w_p"], [0.5, 1.5, 0.5, 1.5])
+    np.testing.assert_allclose(h["cost"], -h["+gof"])
+    assert opt["cost"] == h["cost"].min()
+    # the same simulations through the SimGroup directly
+    sg = p.sim_group
+    sg.N = 4
+    sg.param_lists["G"] = h["G"].values
+    sg.param_lists["w_p"] = np.repeat(h["w_p"].values[:, None], N, axis=1)
+    sg.param_lists["J_N"] = np.full((4, N), 0.15)
+    sg._set_default_params()
+    sg.run()
+    s = sg.score(p.emp_fc_tril, p.emp_fcd_tril)
+    np.testing.assert_allclose(s["+gof"].values, h["+gof"].values, rtol=0, atol=1e-12)
+
+
+def test_fic_penalty_enters_the_cost(oracle_core):
+    p = _problem({"G": (0.5, 3.0)}, do_fic=True)
+    F, scores = p.evaluate(np.array([[0.2], [0.8]]))
+    assert "-fic_penalty" in scores.columns
+    np.testing.assert_allclose(F, -scores["+gof"].values - scores["-fic_penalty"].values)
+
+
+def test_cmaes_optimizer_end_to_end(oracle_core):
+    from cubnm_amd.optimize import CMAESOptimizer
+    p = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5)})
+    cm = CMAESOptimizer(popsize=4, n_iter=2, seed=3)
+    cm.setup_problem(p)
+    opt = cm.optimize()
+    h = cm.history
+    assert len(h) == 12 and sorted(h["gen"].unique().tolist()) == [0, 1, 2]
+    assert ((h["G"] >= 0.5) & (h["G"] <= 3.0) & (h["w_p"] >= 0.5) & (h["w_p"] <= 1.5)).all()
+    assert opt["cost"] == h["cost"].min() and len(cm.opt_history) == 3
