@@ -142,11 +142,17 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
 #pragma unroll
                 for (int sl = 0; sl < NS; sl++) M::pre(mc, s[sl], gp, rp[sl], n0[sl], n1[sl], tp[sl]);
                 const double* xb = xs + buf * buf_stride;
-                v4d acc[TPW][NX];
+                // two accumulators per product (even and odd source
+                // fragments), added at the end: two independent MFMA chains
+                // instead of one dependent one
+                v4d acc[TPW][NX], acc2[TPW][NX];
 #pragma unroll
                 for (int t = 0; t < TPW; t++)
 #pragma unroll
-                    for (int x = 0; x < NX; x++) acc[t][x] = v4d{0.0, 0.0, 0.0, 0.0};
+                    for (int x = 0; x < NX; x++) {
+                        acc[t][x] = v4d{0.0, 0.0, 0.0, 0.0};
+                        acc2[t][x] = v4d{0.0, 0.0, 0.0, 0.0};
+                    }
                 const double* ab = abase + (long long)(wave * TPW) * KT * 64 + lane;
                 const double* bb = xb + lane;
                 if constexpr (KTC > 0) {
@@ -157,8 +163,14 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                         for (int x = 0; x < NX; x++) {
                             const double b = bb[x * KN * 16 + kt * 64];
 #pragma unroll
-                            for (int t = 0; t < TPW; t++)
-                                acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[(t * KT + kt) * 64], b, acc[t][x], 0, 0, 0);
+                            for (int t = 0; t < TPW; t++) {
+                                if (kt & 1)
+                                    acc2[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[(t * KT + kt) * 64], b,
+                                                                                      acc2[t][x], 0, 0, 0);
+                                else
+                                    acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[(t * KT + kt) * 64], b,
+                                                                                     acc[t][x], 0, 0, 0);
+                            }
                         }
                     }
                 } else {
@@ -182,7 +194,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
 #pragma unroll
                             for (int t = 0; t < TPW; t++) {
                                 acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0[x], acc[t][x], 0, 0, 0);
-                                acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1[x], acc[t][x], 0, 0, 0);
+                                acc2[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1[x], acc2[t][x], 0, 0, 0);
                             }
                         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -191,6 +203,10 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                         for (int x = 0; x < NX; x++) { b0[x] = b2[x]; b1[x] = b3[x]; }
                     }
                 }
+#pragma unroll
+                for (int t = 0; t < TPW; t++)
+#pragma unroll
+                    for (int x = 0; x < NX; x++) acc[t][x] += acc2[t][x];
                 double* xn = xs + (buf ^ 1) * buf_stride;
                 const bool last = sample && (inner_i == a.inner_it - 1);
 #pragma unroll

@@ -95,8 +95,12 @@ int cubnm_run_device(const double* d_SC, const int* d_SC_indices, const double* 
  * run_simulations takes one model, node count and duration per call): many
  * groups, each a complete run_simulations request (its own model, node
  * count, duration, config, SCs and parameters), run in one call.  Each group
- * gets its own session and stream; all groups' kernels are queued before any
- * is waited for, costliest first, so they share the GPU.  Host buffers, with
+ * gets its own session; all groups are staged before any kernel runs.  Groups
+ * on the batched (shared-SC) integrator launch on their own streams; all
+ * groups on the per-simulation integrator, any model, run as ONE launch with
+ * their simulations ordered longest first.  With "verbose" in a group's
+ * config, progress (BOLD volumes written, simulations completed) is printed
+ * to stderr while the launch runs.  Host buffers, with
  * the same layouts and in-place parameter write-back as cubnm_run.  Sessions
  * persist between calls (group i reuses session i, and its noise when the
  * seed, node count and segment agree).  Separate from the single-request
