@@ -266,6 +266,12 @@ PyObject* py_run_groups(PyObject*, PyObject* args) {
         PyObject* dct = result_dict(*calls[i]);
         if (!dct) return nullptr;
         PyList_SET_ITEM(list.p, i, dct);
+        // integration kernel time of the launch that ran this request (ms)
+        double t[6];
+        if (cubnm_group_timings((int)i, t) == 0) {
+            Ref im(PyFloat_FromDouble(t[2]));
+            if (!im.p || PyDict_SetItemString(dct, "integrate_ms", im.p) != 0) return nullptr;
+        }
     }
     return list.release();
 }

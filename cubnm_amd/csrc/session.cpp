@@ -184,6 +184,7 @@ struct Session {
 
 Session g_s;
 std::vector<std::unique_ptr<Session>> g_pool;   // sessions of the grouped entry point
+int g_last_groups = 0;                           // group count of the last cubnm_run_groups call
 int g_force_integrator = -1;  // -1 auto, 0 general, 1 batched
 int g_fc_mode = 0;            // 0 auto (LDS kernel when a simulation fits), 1 FP64 Gram, 2 split-BF16 Gram
 int g_general_ksplit = 0;     // partitions of the general kernel's coupling sum (0 auto)
@@ -876,6 +877,15 @@ int cubnm_last_timings(double* out6) {
     return 0;
 }
 
+int cubnm_group_timings(int group, double* out6) {
+    if (group < 0 || group >= g_last_groups || group >= (int)g_pool.size() || !g_pool[group]) {
+        g_err = "no such group in the last cubnm_run_groups call";
+        return -1;
+    }
+    for (int i = 0; i < 6; i++) out6[i] = g_pool[group]->timings[i];
+    return 0;
+}
+
 int cubnm_prepare(const char* model_name, int n_config, const char* const* keys, const char* const* values,
                   int ext_out, int states_ts, int noise_out, int do_delay, int force_reinit, int use_cpu,
                   int N_SIMS, int nodes, int N_SCs, int time_steps, int BOLD_TR, int states_sampling,
@@ -930,6 +940,7 @@ int cubnm_prepare_groups(int n_groups, cubnm_group* groups) {
 int cubnm_run_groups(int n_groups, cubnm_group* groups) {
     int staged = 0;
     std::vector<int> order(std::max(0, n_groups));
+    g_last_groups = 0;
     try {
         if (n_groups <= 0 || (int)g_pool.size() < n_groups) throw std::runtime_error("cubnm_prepare_groups first");
         for (int i = 0; i < n_groups; i++)
@@ -1120,6 +1131,7 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             }
         }
         if (verbose) std::fprintf(stderr, "cubnm_run_groups: all %d groups done at %.1f s\n", n_groups, now_s() - t0);
+        g_last_groups = n_groups;
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

@@ -130,6 +130,10 @@ typedef struct cubnm_group {
 
 int cubnm_prepare_groups(int n_groups, cubnm_group* groups);
 int cubnm_run_groups(int n_groups, cubnm_group* groups);
+/* Timings of group `group` of the last cubnm_run_groups call, as
+ * cubnm_last_timings; [2] is the integration kernel time of the launch that
+ * ran the group (the one merged launch for per-simulation groups). */
+int cubnm_group_timings(int group, double* out6);
 
 /* Timings of the last run (milliseconds): [0] parameter prep on host,
  * [1] host->device copies, [2] integration kernel, [3] FC/FCD kernel,

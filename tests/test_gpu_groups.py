@@ -41,6 +41,8 @@ def test_groups_equal_single_requests():
     for r, g in zip(reqs, grouped):
         a = cases.copy_args(r)
         single = _core.run_simulations(*a)
+        # grouped results also carry the integration time of their launch
+        assert g.pop("integrate_ms") > 0
         assert set(g) == set(single)
         for k in single:
             if k in ("init_time", "run_time"):
