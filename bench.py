@@ -350,6 +350,7 @@ def run_c5(args):
     # lower bound on the FC/FCD stage's rate
     fcd_flops = 2.0 * (n_w * (n_w - 1) // 2) * pairs * 3 * S
     work = S * N * dur
+    int_flops = work * ops_per_sim_node_second(N, 0.1, 1.0, TR)
     line = {
         "metric": METRIC, "value": round(work / wall, 1), "unit": "sim-node-s/s", "n_gpus": 1, "steps": 1,
         "warmup": 1, "ms_per_step": round(1000 * wall, 1), "higher_is_better": True, "scaling": "weak",
@@ -362,7 +363,13 @@ def run_c5(args):
                    "fcd_bf16x3_tflops_lower_bound": round(fcd_flops / (t["fc_fcd_ms"] / 1000) / 1e12, 1),
                    "check_sims": C, "fc_max_abs_err_vs_fp64_host": fc_err,
                    "fcd_max_abs_err_vs_fp64_host": fcd_err},
-        "roofline": None, "cpu_baseline": None,
+        # the dominant kernel is the per-simulation integrator: algorithmic
+        # FP64 ops (rWW, N=400) over its launch time
+        "roofline": {"bound": "mfma", "achieved": round(int_flops / (t["integrate_ms"] / 1000) / 1e12, 3),
+                     "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": round(int_flops / (t["integrate_ms"] / 1000) / 1e12 / 78.6, 4), "traffic": None,
+                     "kernel": "integrate_general_kernel", "kernel_ms": round(t["integrate_ms"], 1)},
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

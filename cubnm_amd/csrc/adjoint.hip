@@ -104,19 +104,30 @@ __global__ void __launch_bounds__(MAXT) rww_fwd_kernel(AdjArgs a, RWW::C c, BWCo
     if (act) conn[j] = s[4];
     double* bold = a.bold + (long long)sim * a.bold_len * N;
     int bold_i = 0;
+    int sh_t = 0, sh_j = 0;
+    double pn0 = 0.0, pn1 = 0.0;  // noise of the next inner step
+    if (own) {
+        noise_shuffle(a, 0, jj, &sh_t, &sh_j);
+        noise_load(a, sh_t, sh_j, 0, &pn0, &pn1);
+    }
     for (int bw_i = 0; bw_i < a.bw_it; bw_i++) {
         if (bw_i % a.ck == 0 && act) {
             double* ck = a.ckpt + ((long long)sim * a.n_ck + bw_i / a.ck) * 6 * N;
             ck[0 * N + j] = s[4]; ck[1 * N + j] = s[5];
             ck[2 * N + j] = bx; ck[3 * N + j] = bf; ck[4 * N + j] = bn; ck[5 * N + j] = bq;
         }
-        int sh_t = 0, sh_j = 0;
-        if (own) noise_shuffle(a, bw_i, jj, &sh_t, &sh_j);
+        // shuffles of the next BW step, for the noise prefetch of its first step
+        int nsh_t = sh_t, nsh_j = sh_j;
+        if (own && bw_i + 1 < a.bw_it) noise_shuffle(a, bw_i + 1, jj, &nsh_t, &nsh_j);
         for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
-            // this step's noise, issued before the barrier so that its latency
-            // overlaps the coupling sum (the barriers order LDS only)
-            double n0 = 0.0, n1 = 0.0;
-            if (own) noise_load(a, sh_t, sh_j, inner_i, &n0, &n1);
+            // this step's noise was loaded one step ahead; load the next
+            // step's now, so that its latency overlaps a whole step (the
+            // barriers order LDS only)
+            const double n0 = pn0, n1 = pn1;
+            if (own) {
+                if (inner_i + 1 < a.inner_it) noise_load(a, sh_t, sh_j, inner_i + 1, &pn0, &pn1);
+                else if (bw_i + 1 < a.bw_it) noise_load(a, nsh_t, nsh_j, 0, &pn0, &pn1);
+            }
             lds_barrier();
             double g = 0.0;
             if (a.sc_lds) {
@@ -136,6 +147,7 @@ __global__ void __launch_bounds__(MAXT) rww_fwd_kernel(AdjArgs a, RWW::C c, BWCo
             lds_barrier();
             if (act) conn[j] = s[4];
         }
+        sh_t = nsh_t; sh_j = nsh_j;
         if (own) bw_step(bw, s[4], bx, bf, bn, bq);
         if ((bw_i + 1) % a.TR_iters == 0) {
             if (act && bold_i < a.bold_len) bold[(long long)bold_i * N + j] = bold_signal(bw, bn, bq);
@@ -189,13 +201,23 @@ __global__ void __launch_bounds__(MAXT) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
         }
         __syncthreads();
         if (act) conn[j] = s[4];
+        // noise loaded one inner step ahead, as in the forward kernel
+        int sh_t = 0, sh_j = 0;
+        double pn0 = 0.0, pn1 = 0.0;
+        if (own) {
+            noise_shuffle(a, b0, jj, &sh_t, &sh_j);
+            noise_load(a, sh_t, sh_j, 0, &pn0, &pn1);
+        }
         for (int b = b0; b < b1; b++) {
             double* sb = scr + (long long)(b - b0) * per_bw * N;
-            int sh_t = 0, sh_j = 0;
-            if (own) noise_shuffle(a, b, jj, &sh_t, &sh_j);
+            int nsh_t = sh_t, nsh_j = sh_j;
+            if (own && b + 1 < b1) noise_shuffle(a, b + 1, jj, &nsh_t, &nsh_j);
             for (int i = 0; i < a.inner_it; i++) {
-                double n0 = 0.0, n1 = 0.0;
-                if (own) noise_load(a, sh_t, sh_j, i, &n0, &n1);
+                const double n0 = pn0, n1 = pn1;
+                if (own) {
+                    if (i + 1 < a.inner_it) noise_load(a, sh_t, sh_j, i + 1, &pn0, &pn1);
+                    else if (b + 1 < b1) noise_load(a, nsh_t, nsh_j, 0, &pn0, &pn1);
+                }
                 lds_barrier();
                 double g = 0.0;
                 if (a.sc_lds) {
@@ -220,33 +242,36 @@ __global__ void __launch_bounds__(MAXT) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
             double* bwst = sb + (long long)3 * a.inner_it * N;
             if (act) { bwst[0 * N + j] = bx; bwst[1 * N + j] = bf; bwst[2 * N + j] = bn; bwst[3 * N + j] = bq; bwst[4 * N + j] = s[4]; }
             if (own) bw_step(bw, s[4], bx, bf, bn, bq);
+            sh_t = nsh_t; sh_j = nsh_j;
         }
-        // ---- backwards through the segment.  The scratch entries and noise
-        // of the next inner step back are loaded one step ahead (pre*), and
-        // the BW-step state one BW step ahead (pb*), so their latency overlaps
-        // the current step; the barriers order LDS only.
-        int nsh_t = 0, nsh_j = 0;
-        double preSE = 0, preSI = 0, preg = 0, pren0 = 0, pren1 = 0;
+        // ---- backwards through the segment.  Step t = (b - b0) * inner_it + i
+        // runs from T-1 down to 0; the scratch entries and noise of step t-2
+        // are loaded while step t is processed (two steps ahead: these are
+        // HBM reads of data written a segment earlier), the BW-step state one
+        // BW step ahead; the barriers order LDS only.
+        const int T = (b1 - b0) * a.inner_it;
+        int fb = -1, fsh_t = 0, fsh_j = 0;  // block and noise shuffles of the last fetch
+        auto fetch = [&](int t, double* v) {  // v = S_E, S_I, g, noise 0, noise 1 of step t
+            const int bb = b0 + t / a.inner_it, ii = t % a.inner_it;
+            if (bb != fb) { noise_shuffle(a, bb, jj, &fsh_t, &fsh_j); fb = bb; }
+            const double* sbx = scr + (long long)(bb - b0) * per_bw * N;
+            v[0] = sbx[(3 * ii) * N + jj]; v[1] = sbx[(3 * ii + 1) * N + jj]; v[2] = sbx[(3 * ii + 2) * N + jj];
+            noise_load(a, fsh_t, fsh_j, ii, &v[3], &v[4]);
+        };
+        double p1[5] = {0, 0, 0, 0, 0}, p2[5] = {0, 0, 0, 0, 0};  // steps t and t-1
         double pbx = 0, pbf = 1, pbn = 1, pbq = 1, pbu = 0;
         if (own) {
-            noise_shuffle(a, b1 - 1, jj, &nsh_t, &nsh_j);
-            const double* sb = scr + (long long)(b1 - 1 - b0) * per_bw * N;
-            const int i = a.inner_it - 1;
-            preSE = sb[(3 * i) * N + jj]; preSI = sb[(3 * i + 1) * N + jj]; preg = sb[(3 * i + 2) * N + jj];
-            noise_load(a, nsh_t, nsh_j, i, &pren0, &pren1);
-            const double* bwst = sb + (long long)3 * a.inner_it * N;
+            fetch(T - 1, p1);
+            if (T >= 2) fetch(T - 2, p2);
+            const double* bwst = scr + (long long)(b1 - 1 - b0) * per_bw * N + (long long)3 * a.inner_it * N;
             pbx = bwst[0 * N + jj]; pbf = bwst[1 * N + jj]; pbn = bwst[2 * N + jj]; pbq = bwst[3 * N + jj];
             pbu = bwst[4 * N + jj];
         }
         for (int b = b1 - 1; b >= b0; b--) {
-            const int sh_t = nsh_t, sh_j = nsh_j;
-            const double* sb = scr + (long long)(b - b0) * per_bw * N;
-            const double* sbn = scr + (long long)(b - 1 - b0) * per_bw * N;  // used only when b > b0
             if (own) {
-                if (b > b0) noise_shuffle(a, b - 1, jj, &nsh_t, &nsh_j);
                 const double x = pbx, f = pbf, nu = pbn, q = pbq, u = pbu;
                 if (b > b0) {
-                    const double* bwst = sbn + (long long)3 * a.inner_it * N;
+                    const double* bwst = scr + (long long)(b - 1 - b0) * per_bw * N + (long long)3 * a.inner_it * N;
                     pbx = bwst[0 * N + jj]; pbf = bwst[1 * N + jj]; pbn = bwst[2 * N + jj]; pbq = bwst[3 * N + jj];
                     pbu = bwst[4 * N + jj];
                 }
@@ -279,17 +304,11 @@ __global__ void __launch_bounds__(MAXT) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
             for (int i = a.inner_it - 1; i >= 0; i--) {
                 double eE = 0.0, eI = 0.0, muE = 0.0, muI = 0.0, rE = 0.0;
                 if (own) {
-                    const double SE = preSE, SI = preSI, g = preg, n0 = pren0, n1 = pren1;
-                    if (i > 0) {
-                        preSE = sb[(3 * (i - 1)) * N + jj]; preSI = sb[(3 * (i - 1) + 1) * N + jj];
-                        preg = sb[(3 * (i - 1) + 2) * N + jj];
-                        noise_load(a, sh_t, sh_j, i - 1, &pren0, &pren1);
-                    } else if (b > b0) {
-                        const int il = a.inner_it - 1;
-                        preSE = sbn[(3 * il) * N + jj]; preSI = sbn[(3 * il + 1) * N + jj];
-                        preg = sbn[(3 * il + 2) * N + jj];
-                        noise_load(a, nsh_t, nsh_j, il, &pren0, &pren1);
-                    }
+                    const double SE = p1[0], SI = p1[1], g = p1[2], n0 = p1[3], n1 = p1[4];
+#pragma unroll
+                    for (int q = 0; q < 5; q++) p1[q] = p2[q];
+                    const int t = (b - b0) * a.inner_it + i;
+                    if (t >= 2) fetch(t - 2, p2);
                     const double IE = c.w_E__I_0 + wp * JN * SE + g * G * JN - wIE * SI;
                     const double II = c.w_I__I_0 + JN * SE - c.w_II * SI;
                     const double aE = c.a_E * IE - c.b_E, aI = c.a_I * II - c.b_I;
