@@ -292,6 +292,23 @@ const void* model_consts(const Session& s) {
     }
 }
 
+// While set (cubnm_prepare_groups), ensure_noise records the keys it would
+// generate instead of generating them, so that distinct segments can be
+// generated in parallel; each is still one uninterrupted std::mt19937 stream.
+bool g_noise_deferred = false;
+std::vector<NoiseKey> g_deferred_keys;
+
+// Host part of a noise entry: the segment's normals and the generator state
+// after them (bnm.cu:1207-1263).
+void generate_noise(const NoiseKey& key, NoiseEntry& e) {
+    std::mt19937 gen(key.seed);
+    std::normal_distribution<double> nd(0, 1);
+    const long long size = (long long)key.nodes * key.noise_bw_it * key.inner_it * key.n_noise;
+    e.noise.resize(size);
+    for (long long i = 0; i < size; i++) e.noise[i] = nd(gen);
+    e.after = gen;
+}
+
 // Noise segment and per-repeat shuffles from one std::mt19937 stream.
 void ensure_noise(Session& s, bool force) {
     const int n_noise = s.model->n_noise;
@@ -304,13 +321,14 @@ void ensure_noise(Session& s, bool force) {
         std::shared_ptr<NoiseEntry> e;
         auto it = g_noise_cache.find(key);
         if (!force && it != g_noise_cache.end()) e = it->second.lock();
+        if (!e && g_noise_deferred && !force) {
+            g_deferred_keys.push_back(key);
+            s.noise_e.reset();
+            return;
+        }
         if (!e) {
             e = std::make_shared<NoiseEntry>();
-            std::mt19937 gen(s.sim_seed);
-            std::normal_distribution<double> nd(0, 1);
-            e->noise.resize(s.noise_size);
-            for (long long i = 0; i < s.noise_size; i++) e->noise[i] = nd(gen);
-            e->after = gen;
+            generate_noise(key, *e);
             e->d_noise.ensure(sizeof(double) * s.noise_size);
             check(hipMemcpy(e->d_noise.p, e->noise.data(), sizeof(double) * s.noise_size, hipMemcpyHostToDevice),
                   "noise H2D");
@@ -916,9 +934,12 @@ int cubnm_run(const double* SC, const int* SC_indices, const double* SC_dist, do
 }
 
 int cubnm_prepare_groups(int n_groups, cubnm_group* groups) {
+    g_deferred_keys.clear();
     try {
         if (n_groups <= 0) throw std::runtime_error("need at least one group");
         while ((int)g_pool.size() < n_groups) g_pool.emplace_back(new Session());
+        // 1. sessions, with the generation of missing noise segments deferred
+        g_noise_deferred = true;
         for (int i = 0; i < n_groups; i++) {
             cubnm_group& g = groups[i];
             try {
@@ -927,9 +948,38 @@ int cubnm_prepare_groups(int n_groups, cubnm_group* groups) {
                                 g.time_steps, g.BOLD_TR, g.states_sampling, g.sim_seed, g.dt, g.bw_dt, &g.dims);
                 g_pool[i]->tpw = g.batched_tpw;
             } catch (const std::exception& e) {
+                g_noise_deferred = false;
                 throw std::runtime_error("group " + std::to_string(i) + ": " + e.what());
             }
         }
+        g_noise_deferred = false;
+        // 2. distinct missing segments generated in parallel (one thread per
+        //    segment, each its own uninterrupted stream), uploaded in order
+        std::sort(g_deferred_keys.begin(), g_deferred_keys.end());
+        g_deferred_keys.erase(std::unique(g_deferred_keys.begin(), g_deferred_keys.end()), g_deferred_keys.end());
+        const double t0 = now_s();
+        std::vector<std::shared_ptr<NoiseEntry>> fresh(g_deferred_keys.size());
+        for (auto& e : fresh) e = std::make_shared<NoiseEntry>();
+        {
+            std::vector<std::thread> workers;
+            for (size_t k = 0; k < fresh.size(); k++)
+                workers.emplace_back([&, k]() { generate_noise(g_deferred_keys[k], *fresh[k]); });
+            for (auto& w : workers) w.join();
+        }
+        for (size_t k = 0; k < fresh.size(); k++) {
+            const size_t bytes = sizeof(double) * fresh[k]->noise.size();
+            fresh[k]->d_noise.ensure(bytes);
+            check(hipMemcpy(fresh[k]->d_noise.p, fresh[k]->noise.data(), bytes, hipMemcpyHostToDevice), "noise H2D");
+            g_noise_cache[g_deferred_keys[k]] = fresh[k];
+        }
+        groups[0].dims.init_time += now_s() - t0;  // the shared generation is booked on the first group
+        // 3. every session takes its segment from the cache and draws its shuffles
+        for (int i = 0; i < n_groups; i++) {
+            const double t1 = now_s();
+            ensure_noise(*g_pool[i], false);
+            groups[i].dims.init_time += now_s() - t1;
+        }
+        g_deferred_keys.clear();
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();
