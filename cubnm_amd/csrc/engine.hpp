@@ -50,6 +50,7 @@ struct IntegrateArgs {
     BWConsts bw;
     int sc_lds;               // general kernel: SC staged in LDS (rows padded by one)
     int ksplit;               // general kernel: partitions of the coupling sum over k (>= 1)
+    int pair;                 // general kernel: coupling lanes sum two adjacent columns (16-byte loads)
 };
 
 struct FcArgs {
@@ -69,12 +70,28 @@ struct FcArgs {
 };
 
 // Host-side launchers (defined in the .hip files).
-hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* model_consts, hipStream_t s);
+// latency_bound: the request's simulations share SCs or are few (pair mode
+// allowed; see general_pair_ok)
+hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* model_consts, hipStream_t s,
+                                    bool latency_bound);
 // general kernel geometry: lanes per k partition, partitions (want 0 = auto),
 // LDS bytes of a workgroup; several requests of one model in one launch
 int general_np(int nodes);
 int general_ksplit(int nodes, int want);
-size_t general_lds_bytes(int model, int nodes, int np, int ksplit, bool sc_staged);
+size_t general_lds_bytes(int model, int nodes, int np, int ksplit, bool sc_staged, bool pair, int threads);
+// pair mode of the general kernel: lanes per partition (two adjacent source
+// columns per lane), its eligibility, and the doubles of the partial-sum
+// region (the k partitions' or the pair partitions', whichever is larger)
+__host__ __device__ inline int general_npc(int nodes) { return 64 * ((nodes / 2 + 63) / 64); }
+bool general_pair_ok(const IntegrateArgs& a, int threads, bool latency_bound);
+__host__ __device__ inline int general_red_doubles(bool osc, int nodes, int np, int ks, bool pair, int threads) {
+    int r = (ks - 1) * 2 * np;
+    if (pair) {
+        const int pr = (threads / general_npc(nodes)) * nodes * (osc ? 2 : 1);
+        r = r > pr ? r : pr;
+    }
+    return r;
+}
 size_t model_consts_bytes(int model);
 size_t model_consts_stride();
 hipError_t launch_integrate_general_multi(const IntegrateArgs* d_args, const void* d_consts, const int* d_models,

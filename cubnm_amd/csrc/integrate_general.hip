@@ -19,6 +19,7 @@
 // without delay use sin(x_k - x_j) = sin x_k cos x_j - cos x_k sin x_j: two
 // sums over published sin/cos instead of nodes^2 sine evaluations per step.
 #include "engine.hpp"
+#include <cstdint>
 
 namespace cubnm {
 
@@ -75,6 +76,45 @@ __device__ __forceinline__ void coupling_sum(P sc, const int ldsc, const double*
     }
 }
 
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+// Pair mode: this lane's two adjacent columns (c, c+1) of
+// sum_{k0 <= k < k1} SC[k][c] * x_k, with 16-byte loads (ld2: row stride in
+// column pairs), in ascending k.  Halves the load instructions of the
+// single-column form, which bound it.
+template <bool OSC>
+__device__ __forceinline__ void pair_sum(const gmem<const double>* sc, const int ld2, const double* x, const int N,
+                                         const int k0, const int k1, double& ga, double& gb, double& ga2,
+                                         double& gb2) {
+    const gmem<const dbl2>* p = (const gmem<const dbl2>*)sc;
+    constexpr int U = 4;
+    int k = k0;
+    for (; k + U <= k1; k += U) {
+        dbl2 w[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) w[u] = p[(long long)u * ld2];
+        p += (long long)U * ld2;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            ga += w[u].x * x[k + u];
+            gb += w[u].y * x[k + u];
+            if constexpr (OSC) {
+                ga2 += w[u].x * x[N + k + u];
+                gb2 += w[u].y * x[N + k + u];
+            }
+        }
+    }
+    for (; k < k1; k++, p += ld2) {
+        const dbl2 w = *p;
+        ga += w.x * x[k];
+        gb += w.y * x[k];
+        if constexpr (OSC) {
+            ga2 += w.x * x[N + k];
+            gb2 += w.y * x[N + k];
+        }
+    }
+}
+
 }  // namespace
 
 template <class M>
@@ -99,9 +139,23 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
     const int kc = (N + ks - 1) / ks;
     const int k0 = wnode ? min(N, part * kc) : 0;
     const int k1 = wnode ? min(N, k0 + kc) : 0;
+    // pair mode (SC in global memory, no delay, even N): the coupling is
+    // summed by P partitions of npc lanes, lane t of partition p taking
+    // columns 2t and 2t+1 over partition p's source rows; node lanes then add
+    // the P partial sums from LDS in partition order
+    const bool pair = a.pair != 0;
+    const int npc = general_npc(N);
+    const int P = pair ? (int)blockDim.x / npc : 1;
+    const int pp = threadIdx.x / npc;
+    const int pc = 2 * (threadIdx.x - pp * npc);
+    const bool pact = pair && pp < P && pc < N;
+    const int pkc = (N + P - 1) / P;
+    const int pk0 = pact ? min(N, pp * pkc) : 0;
+    const int pk1 = pact ? min(N, pk0 + pkc) : 0;
     extern __shared__ double lds[];
-    double* red = lds;                       // [ks-1][2][np] partial sums of partitions 1..ks-1
-    double* conn = lds + (ks - 1) * 2 * np;  // previous-step coupling state: [N], or sin [N] then cos [N]
+    // partial sums: k partitions 1..ks-1 ([ks-1][2][np]) or pair partitions ([2][P][N])
+    double* red = lds;
+    double* conn = lds + general_red_doubles(M::is_osc, N, np, ks, pair, blockDim.x);
     const int nc = M::is_osc ? 2 * N : N;
     const gmem<const double>* SCg = as_global(a.SC) + (long long)a.SC_indices[sim] * N * N;
     // SC in LDS (rows padded by one double) when it fits: the coupling loop
@@ -179,10 +233,31 @@ __device__ __forceinline__ void general_body(const IntegrateArgs& a, const typen
                 }
             } else if (a.sc_lds) {
                 coupling_sum<M::is_osc>(scs + (long long)k0 * ldsc + jj, ldsc, conn, N, k0, k1, g, g2);
+            } else if (pair) {
+                double ga = 0.0, gb = 0.0, ga2 = 0.0, gb2 = 0.0;
+                pair_sum<M::is_osc>(SCg + (long long)pk0 * N + (pact ? pc : 0), N / 2, conn, N, pk0, pk1, ga, gb,
+                                    ga2, gb2);
+                if (pact) {
+                    red[(long long)pp * N + pc] = ga;
+                    red[(long long)pp * N + pc + 1] = gb;
+                    if constexpr (M::is_osc) {
+                        red[(long long)(P + pp) * N + pc] = ga2;
+                        red[(long long)(P + pp) * N + pc + 1] = gb2;
+                    }
+                }
+                lds_barrier();
+                if (act) {
+                    g = red[jj];
+                    for (int p = 1; p < P; p++) g += red[(long long)p * N + jj];
+                    if constexpr (M::is_osc) {
+                        g2 = red[(long long)P * N + jj];
+                        for (int p = 1; p < P; p++) g2 += red[(long long)(P + p) * N + jj];
+                    }
+                }
             } else {
                 coupling_sum<M::is_osc>(SCg + (long long)k0 * ldsc + jj, ldsc, conn, N, k0, k1, g, g2);
             }
-            if (ks > 1) {
+            if (ks > 1 && !pair) {
                 if (part > 0) {
                     red[(long long)(part - 1) * 2 * np + j] = g;
                     red[(long long)((part - 1) * 2 + 1) * np + j] = g2;
@@ -323,8 +398,9 @@ __global__ void __launch_bounds__(1024) integrate_general_multi_kernel(const Int
 // LDS bytes of one general-kernel workgroup: partial sums of k partitions
 // 1..ksplit-1 (two per lane), the previous-step coupling state (sin and cos
 // for oscillators) and, when staged, SC with rows padded by one double.
-size_t general_lds_bytes(int model, int nodes, int np, int ksplit, bool sc_staged) {
-    size_t d = (size_t)(ksplit - 1) * 2 * np + (size_t)(model == MODEL_KURAMOTO ? 2 : 1) * nodes;
+size_t general_lds_bytes(int model, int nodes, int np, int ksplit, bool sc_staged, bool pair, int threads) {
+    size_t d = (size_t)general_red_doubles(model == MODEL_KURAMOTO, nodes, np, ksplit, pair, threads) +
+               (size_t)(model == MODEL_KURAMOTO ? 2 : 1) * nodes;
     if (sc_staged) d += (size_t)nodes * (nodes + 1);
     return d * sizeof(double);
 }
@@ -344,23 +420,38 @@ int general_ksplit(int nodes, int want) {
     return std::max(1, std::min({4, kmax, nodes / 25}));
 }
 
-// LDS bytes and SC staging of a request on the general integrator: staged
-// when it leaves room for two workgroups per CU, or when there are too few
-// simulations to fill the CUs anyway (latency-bound).
-static size_t general_lds(int model, const IntegrateArgs& a, int* sc_lds) {
-    const int np = general_np(a.nodes);
-    const size_t with_sc = general_lds_bytes(model, a.nodes, np, a.ksplit, true);
-    *sc_lds = with_sc <= 64 * 1024 || (with_sc <= 150 * 1024 && a.N_SIMS <= 512);
-    return *sc_lds ? with_sc : general_lds_bytes(model, a.nodes, np, a.ksplit, false);
+// Pair mode for a request on a launch of `threads` lanes: SC read from global
+// memory (not staged, no delay), an even node count of at least 96, the SC
+// 16-byte aligned, room for two pair partitions, and a launch that is bound
+// by load instructions rather than HBM (`latency_bound`: a shared SC or few
+// simulations).  Measured on MI355X, rWW per step: N=400 one simulation 21 ->
+// 12.7 us, 256 simulations sharing one SC 21.4 -> 13.1 us, N=200 one
+// simulation 6.6 -> 4.3 us; but 256 simulations with one SC each (HBM-bound)
+// 43-45 -> 48 us, N=200 x 512 21.5 -> 23 us.
+bool general_pair_ok(const IntegrateArgs& a, int threads, bool latency_bound) {
+    return latency_bound && !a.do_delay && !a.sc_lds && a.nodes % 2 == 0 && a.nodes >= 96 &&
+           reinterpret_cast<uintptr_t>(a.SC) % 16 == 0 && 2 * general_npc(a.nodes) <= threads;
+}
+
+// SC staging, pair mode and LDS bytes of a request on the general integrator:
+// SC staged when it leaves room for two workgroups per CU, or when there are
+// too few simulations to fill the CUs anyway (latency-bound).
+static size_t general_lds(int model, IntegrateArgs& b, int threads, bool latency_bound) {
+    const int np = general_np(b.nodes);
+    const size_t with_sc = general_lds_bytes(model, b.nodes, np, b.ksplit, true, false, threads);
+    b.sc_lds = with_sc <= 64 * 1024 || (with_sc <= 150 * 1024 && b.N_SIMS <= 512);
+    b.pair = general_pair_ok(b, threads, latency_bound);
+    return general_lds_bytes(model, b.nodes, np, b.ksplit, b.sc_lds, b.pair != 0, threads);
 }
 
 template <class M>
-static hipError_t launch_general(int model, const IntegrateArgs& a, const void* consts, hipStream_t st) {
+static hipError_t launch_general(int model, const IntegrateArgs& a, const void* consts, hipStream_t st,
+                                 bool latency_bound) {
     const typename M::C mc = *static_cast<const typename M::C*>(consts);
     IntegrateArgs b = a;
     if (b.ksplit < 1 || b.ksplit * general_np(a.nodes) > 1024) return hipErrorInvalidValue;
     const int threads = b.ksplit * general_np(a.nodes);
-    const size_t lds = general_lds(model, a, &b.sc_lds);
+    const size_t lds = general_lds(model, b, threads, latency_bound);
     hipLaunchKernelGGL(integrate_general_kernel<M>, dim3(a.N_SIMS), dim3(threads), lds, st, b, mc);
     return hipGetLastError();
 }
@@ -396,13 +487,14 @@ hipError_t launch_integrate_general_multi(const IntegrateArgs* d_args, const voi
     return hipGetLastError();
 }
 
-hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* consts, hipStream_t s) {
+hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
+                                    bool latency_bound) {
     if (a.nodes < 1 || a.nodes > 1024) return hipErrorInvalidValue;
     switch (model) {
-        case MODEL_RWW: return launch_general<RWW>(model, a, consts, s);
-        case MODEL_JR: return launch_general<JR>(model, a, consts, s);
-        case MODEL_KURAMOTO: return launch_general<Kuramoto>(model, a, consts, s);
-        case MODEL_WC: return launch_general<WC>(model, a, consts, s);
+        case MODEL_RWW: return launch_general<RWW>(model, a, consts, s, latency_bound);
+        case MODEL_JR: return launch_general<JR>(model, a, consts, s, latency_bound);
+        case MODEL_KURAMOTO: return launch_general<Kuramoto>(model, a, consts, s, latency_bound);
+        case MODEL_WC: return launch_general<WC>(model, a, consts, s, latency_bound);
         default: return hipErrorInvalidValue;
     }
 }

@@ -610,7 +610,9 @@ void launch_integrator(Session& s, const IntegrateArgs& a) {
         check(launch_integrate_batched(s.model->id, a, model_consts(s), s.stream, &launches, s.d_groups.as<int>(),
                                        s.n_groups, s.d_afrag.as<double>(), s.tpw),
               "integrate_batched launch");
-    else check(launch_integrate_general(s.model->id, a, model_consts(s), s.stream), "integrate_general launch");
+    else check(launch_integrate_general(s.model->id, a, model_consts(s), s.stream,
+                                        s.N_SIMS <= 512 || s.N_SIMS >= 4 * s.N_SCs),
+               "integrate_general launch");
     check(hipEventRecord(s.ev[3], s.stream), "event");
     s.timings[5] = launches;
 }
@@ -1056,17 +1058,23 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
         }
         for (auto& kv : buckets) {
             Bucket& b = kv.second;
-            int nmax = 1;
-            for (int i : b.members) nmax = std::max(nmax, g_pool[i]->nodes);
+            int nmax = 1, sims = 0;
+            for (int i : b.members) {
+                nmax = std::max(nmax, g_pool[i]->nodes);
+                sims += g_pool[i]->N_SIMS;
+            }
             b.ks = general_ksplit(nmax, g_general_ksplit);
             b.threads = b.ks * b.np;
             for (int i : b.members) {
                 Session& s = *g_pool[i];
-                // small SCs staged; larger ones read through the caches
-                const size_t with_sc = general_lds_bytes(s.model->id, s.nodes, b.np, b.ks, true);
+                // small SCs staged; larger ones read through the caches, in
+                // pair mode where the request allows it
+                const size_t with_sc = general_lds_bytes(s.model->id, s.nodes, b.np, b.ks, true, false, b.threads);
                 s.iargs.sc_lds = with_sc <= 64 * 1024;
                 s.iargs.ksplit = b.ks;
-                b.lds = std::max(b.lds, general_lds_bytes(s.model->id, s.nodes, b.np, b.ks, s.iargs.sc_lds));
+                s.iargs.pair = general_pair_ok(s.iargs, b.threads, sims <= 512 || s.N_SIMS >= 4 * s.N_SCs);
+                b.lds = std::max(b.lds, general_lds_bytes(s.model->id, s.nodes, b.np, b.ks, s.iargs.sc_lds,
+                                                          s.iargs.pair != 0, b.threads));
             }
         }
         // host copies of the launch tables stay alive until the next call

@@ -73,3 +73,21 @@ def test_groups_match_oracle_and_write_back():
     cases.run_oracle(b)
     np.testing.assert_allclose(a[0][5], b[5], rtol=1e-12, atol=0)
     assert np.all(a[0][5][2] > 0)
+
+
+def test_pair_mode_requests_match_oracle():
+    # 100 and 120 nodes: SCs above the merged launch's 64 KB LDS budget, so the
+    # one per-simulation launch sums their coupling in pair mode
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    reqs = []
+    for model, N, dur, seed in [("rWW", 100, 4, 9), ("Kuramoto", 120, 2, 10)]:
+        sc = cases.synthetic_sc(N, seed=seed)
+        gp, rp = cases.params(model, 2, N, G=[0.4, 0.8], do_fic=False)
+        cfg = cases.model_config(model, TR=0.5, window_size=1, window_step=1, bold_remove_s=1, do_fcd=False,
+                                 do_fic=False)
+        reqs.append(cases.sim_args(model, sc, gp, rp, cfg, 2, duration_s=dur, TR_s=0.5))
+    grouped = _core.core.run_groups([cases.copy_args(r) for r in reqs])
+    for r, g, tol in zip(reqs, grouped, [1e-9, 1e-7]):
+        ref = cases.run_oracle(cases.copy_args(r))
+        assert cases.bold_rel_err(g["sim_bold"], ref["sim_bold"]).max() <= tol, r[0]

@@ -115,6 +115,22 @@ def test_conduction_delay(core, model, request):
     _compare(out, ref, model)
 
 
+@pytest.mark.parametrize("model", ["rWW", "Kuramoto"])
+def test_large_network_pair_mode(core, model):
+    # 144 nodes: SC too large for LDS, so the per-simulation integrator sums the
+    # coupling two columns per lane with 16-byte loads (pair mode)
+    N, S = 144, 2
+    sc = cases.synthetic_sc(N, seed=8)
+    gp, rp = cases.params(model, S, N, G=[0.4, 0.8], do_fic=False)
+    TR = 0.5
+    cfg = cases.model_config(model, TR=TR, window_size=1, window_step=1, bold_remove_s=1, do_fcd=False,
+                             do_fic=False)
+    dur = 4 if model == "rWW" else 3
+    args = cases.sim_args(model, sc, gp, rp, cfg, S, duration_s=dur, TR_s=TR)
+    out, ref, _, _ = _run_both(core, args)
+    _compare(out, ref, model, tol=TOL if model == "rWW" else 1e-7)
+
+
 def test_states_time_series(core, request):
     if request.node.callspec.params["core"] == "batched":
         pytest.skip("state time series run on the general integrator")
