@@ -90,33 +90,34 @@ struct RWW {
         s[5] = fmax(0.0, fmin(1.0, s[5]));
     }
     // The same step in two parts for the batched integrator: `pre` is
-    // everything that does not read this step's coupling input g (the
-    // inhibitory population, and the partial sums of I_E), so it can run
-    // while the coupling product is on the matrix cores; `post` finishes
-    // I_E and the excitatory population.  Operations and their order are
-    // those of `step`.
-    static constexpr int n_tmp = 2;
+    // everything that reads neither this step's coupling input g nor its
+    // noise (I_I, r_I, the deterministic part of dS_I and the partial sums
+    // of I_E), so it can run while the coupling product is on the matrix
+    // cores and the noise loads are in flight; `post` finishes I_E, the
+    // excitatory population and S_I.  The operations are those of `step`;
+    // dS_I adds its noise term last (step adds it first), a difference in
+    // rounding only.
+    static constexpr int n_tmp = 3;
     CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double n1,
                              double* t) {
-        (void)gp; (void)n0;
+        (void)gp; (void)n0; (void)n1;
         t[0] = c.w_E__I_0 + rp[0] * rp[1] * s[4];
         t[1] = rp[2] * s[5];
         s[1] = c.w_I__I_0 + rp[1] * s[4] - c.w_II * s[5];
         const double aI = c.a_I * s[1] - c.b_I;
         s[3] = DIV(aI, 1 - exp(-c.d_I * aI));
-        const double dSI = n1 * rp[3] * c.sqrt_dt + c.dt_gamma_I * s[3] - c.dt_itau_I * s[5];
-        s[5] += dSI;
-        s[5] = fmax(0.0, fmin(1.0, s[5]));
+        t[2] = c.dt_gamma_I * s[3] - c.dt_itau_I * s[5];
     }
     CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
                               double n1, const double* t) {
-        (void)n1;
         s[0] = t[0] + g * gp[0] * rp[1] - t[1];
         const double aE = c.a_E * s[0] - c.b_E;
         s[2] = DIV(aE, 1 - exp(-c.d_E * aE));
         const double dSE = n0 * rp[3] * c.sqrt_dt + c.dt_gamma_E * ((1 - s[4]) * s[2]) - c.dt_itau_E * s[4];
         s[4] += dSE;
         s[4] = fmax(0.0, fmin(1.0, s[4]));
+        s[5] += n1 * rp[3] * c.sqrt_dt + t[2];
+        s[5] = fmax(0.0, fmin(1.0, s[5]));
     }
 };
 
