@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "sim-node-seconds/wall-s (rWW N=100, 60 s); grad-fit vs CMA-ES time-to-loss"
-A100_PUBLISHED = 585000.0        # sim-node-s/s, 32,768 x 100 x 60 s in 5.6 min
+A100_PUBLISHED = 585000.0        # sim-node-s/s: the reference's published A100 run of this grid, 32,768 x 100 x 60 s in 5.6 min
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 (vector and matrix)
 
 
