@@ -117,8 +117,9 @@ def main():
     ap.add_argument("--c4-gens", type=int, default=60, help="CMA-ES generations (c4)")
     ap.add_argument("--c4-steps", type=int, default=40, help="maximum Adam steps (c4)")
     ap.add_argument("--c4-duration", type=float, default=120.0, help="simulated seconds (c4)")
-    ap.add_argument("--c4-lr", type=float, default=0.1,
-                    help="Adam learning rate in the unit cube (c4; 0.1 was the best of 0.02, 0.05, 0.1, 0.2, 0.3)")
+    ap.add_argument("--c4-lr", type=float, default=0.12,
+                    help="Adam learning rate in the unit cube (c4; 0.12 was the best of 0.02, 0.05, 0.1, 0.12, "
+                         "0.15, 0.2, 0.3)")
     ap.add_argument("--c5-sims", type=int, default=8192, help="simulations (c5)")
     ap.add_argument("--c5-check-sims", type=int, default=4,
                     help="simulations whose FC/FCD are recomputed in FP64 on the host (c5)")
