@@ -15,6 +15,11 @@ fails it:
     simulation couples through SC transposed (the FIC K_EE and the
     simulation's coupling use the same orientation).
   * G = 0 decouples nodes: outputs do not depend on SC.
+  * Kuramoto coupling, two nodes without noise, SC[0,1] = a != SC[1,0] = b:
+    a*theta_0 + b*theta_1 - (a + b)*omega*t is conserved (the coupling terms
+    cancel; the transposed SC conserves b*theta_0 + a*theta_1 instead), and
+    the phase difference follows tan(D/2) = tan(D0/2) exp(-G (a + b) t)
+    (a flipped sine argument makes it grow, a lost or doubled G misses it).
   * FC/FCD of a 4-node simulation recomputed by brute force from its BOLD
     with numpy (burn-in rows, window bounds, pair order,
     inter-hemispheric exclusion).
@@ -57,6 +62,26 @@ def test_noise_indexing_kuramoto_random_walk():
     expected = np.pi + np.sqrt(0.1) * sigma * np.cumsum(xi.sum(axis=1), axis=0)
     assert np.abs(theta - np.pi).max() > 1e-2                     # the walk moved
     np.testing.assert_allclose(theta, expected, rtol=0, atol=1e-12)
+
+
+def test_kuramoto_two_node_closed_form():
+    a, b, G, omega = 0.6, 0.2, 0.004, 0.01                # omega in rad/ms, dt = 0.1 ms
+    sc = np.array([[0.0, a], [b, 0.0]])
+    th0 = np.array([0.3, 2.9])
+    gp, rp = cases.params("Kuramoto", 1, 2, G=[G], init_theta=th0, omega=omega, sigma=0.0)
+    cfg = cases.model_config("Kuramoto", do_fc=False, do_fcd=False, bold_remove_s=0, random_init_theta=False)
+    args = cases.sim_args("Kuramoto", sc, gp, rp, cfg, 1, duration_s=2, states_ts=True, states_sampling_s=0.001)
+    out = cases.run_oracle(args)
+    theta = np.unwrap(out["_sim_states"][0, 0].reshape(2000, 2), axis=0)   # phases are kept mod 2 pi
+    t = np.arange(1, 2001, dtype=float)                                   # sample i is after i + 1 ms
+    conserved = a * theta[:, 0] + b * theta[:, 1] - (a + b) * omega * t
+    np.testing.assert_allclose(conserved, a * th0[0] + b * th0[1], rtol=0, atol=1e-12)
+    transposed = b * theta[:, 0] + a * theta[:, 1] - (a + b) * omega * t
+    assert np.abs(transposed - (b * th0[0] + a * th0[1])).max() > 0.5
+    d = theta[:, 1] - theta[:, 0]
+    expected = np.tan((th0[1] - th0[0]) / 2) * np.exp(-G * (a + b) * t)
+    assert d[-1] < 0.02                                                   # synchronised
+    np.testing.assert_allclose(np.tan(d / 2), expected, rtol=2e-3)        # Euler error ~6e-4
 
 
 def _asym_case():
