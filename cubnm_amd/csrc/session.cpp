@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -955,18 +956,33 @@ int cubnm_prepare_groups(int n_groups, cubnm_group* groups) {
             }
         }
         g_noise_deferred = false;
-        // 2. distinct missing segments generated in parallel (one thread per
-        //    segment, each its own uninterrupted stream), uploaded in order
+        // 2. distinct missing segments generated in parallel (each segment one
+        //    uninterrupted stream, segments taken in turn by at most 64
+        //    threads), uploaded in order
         std::sort(g_deferred_keys.begin(), g_deferred_keys.end());
         g_deferred_keys.erase(std::unique(g_deferred_keys.begin(), g_deferred_keys.end()), g_deferred_keys.end());
         const double t0 = now_s();
         std::vector<std::shared_ptr<NoiseEntry>> fresh(g_deferred_keys.size());
         for (auto& e : fresh) e = std::make_shared<NoiseEntry>();
         {
+            const size_t n_keys = fresh.size();
+            const size_t n_workers =
+                std::min(n_keys, (size_t)std::max(1u, std::min(std::thread::hardware_concurrency(), 64u)));
+            std::atomic<size_t> next{0};
+            std::vector<std::exception_ptr> failed(n_workers);
             std::vector<std::thread> workers;
-            for (size_t k = 0; k < fresh.size(); k++)
-                workers.emplace_back([&, k]() { generate_noise(g_deferred_keys[k], *fresh[k]); });
+            for (size_t w = 0; w < n_workers; w++)
+                workers.emplace_back([&, w]() {
+                    try {
+                        for (size_t k = next.fetch_add(1); k < n_keys; k = next.fetch_add(1))
+                            generate_noise(g_deferred_keys[k], *fresh[k]);
+                    } catch (...) {
+                        failed[w] = std::current_exception();
+                    }
+                });
             for (auto& w : workers) w.join();
+            for (auto& f : failed)
+                if (f) std::rethrow_exception(f);
         }
         for (size_t k = 0; k < fresh.size(); k++) {
             const size_t bytes = sizeof(double) * fresh[k]->noise.size();
