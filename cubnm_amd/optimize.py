@@ -14,6 +14,8 @@ rWW FIC penalty hook.  ``GridOptimizer`` (optimize.py:1168-1233) and
 ``CMAESOptimizer`` (optimize.py:1386-1486; here driven by this package's
 own CMA-ES, ``fit.CMAES``, in the unit cube) produce ``history``,
 ``opt_history`` and ``opt`` tables with the reference's columns.
+``batch_optimize`` (optimize.py:1506-1715) runs several CMA-ES optimisations
+with one simulation call per generation.
 """
 import itertools
 import logging
@@ -329,27 +331,98 @@ class CMAESOptimizer:
         self.history.append(res)
         self.opt_history.append(res.loc[res["cost"].argmin()])
 
-    def optimize(self):
+    # One generation at a time, so that batch_optimize can run the
+    # generations of several optimisers as one simulation call.
+    def _start(self):
         from .fit import CMAES
-        rng = np.random.default_rng(self.seed)
+        self._rng = np.random.default_rng(self.seed)
         self.history, self.opt_history = [], []
-        if self.x0 is None:
-            X = rng.uniform(0, 1, (self.popsize, self.problem.ndim))
-            F, scores = self.problem.evaluate(X)
-            F = np.where(np.isnan(F), np.inf, F)
-            self._record(X, F, scores, 0)
-            x0 = X[int(np.argmin(F))]
-        else:
+        self._es = None
+        self._gen = 0
+        if self.x0 is not None:
             x0 = self.problem._get_X(np.asarray(self.x0, float))
-        es = CMAES(x0, self.sigma, self.popsize, seed=self.seed + 1)
-        for g in range(self.n_iter):
-            X = es.ask()
-            F, scores = self.problem.evaluate(X)
-            F = np.where(np.isnan(F), np.inf, F)
-            es.tell(X, F)
-            self._record(X, F, scores, g + 1)
+            self._es = CMAES(x0, self.sigma, self.popsize, seed=self.seed + 1)
+            self._gen = 1
+
+    def _has_next(self):
+        return self._gen <= self.n_iter
+
+    def _ask(self):
+        if self._es is None:      # generation 0: uniform samples in the cube
+            return self._rng.uniform(0, 1, (self.popsize, self.problem.ndim))
+        return self._es.ask()
+
+    def _tell(self, X, F, scores):
+        from .fit import CMAES
+        F = np.where(np.isnan(F), np.inf, F)
+        if self._es is None:
+            self._es = CMAES(X[int(np.argmin(F))], self.sigma, self.popsize, seed=self.seed + 1)
+        else:
+            self._es.tell(X, F)
+        self._record(X, F, scores, self._gen)
+        self._gen += 1
+
+    def _finish(self):
         self.history = pd.concat(self.history, axis=0).reset_index(drop=False)
         self.opt_history = pd.DataFrame(self.opt_history).reset_index(drop=True)
         self.opt = self.history.loc[self.history["cost"].argmin()]
         self.is_fit = True
         return self.opt
+
+    def optimize(self):
+        self._start()
+        while self._has_next():
+            X = self._ask()
+            F, scores = self.problem.evaluate(X)
+            self._tell(X, F, scores)
+        return self._finish()
+
+
+def batch_optimize(optimizers, problems, save=False):
+    """Run several CMA-ES optimisations together (optimize.py:1506-1715):
+    each generation, the candidates of every optimisation still running are
+    simulated in one call through a MultiSimGroup (SCs merged, duplicates
+    shared), then each optimiser scores its own part and takes its next
+    step.  One optimiser for a list of problems, a list of optimisers for
+    one problem, or two lists of the same length.  Optimisers and problems
+    are deep-copied, so the same object may appear several times; every
+    copy draws from its own seeded generators, so each history equals that
+    of the same optimiser run alone on its problem.  Returns the list of
+    fitted optimisers.  ``save=True`` (the reference's default, which writes
+    the optima to disk) is not provided and raises NotImplementedError."""
+    import copy
+    if save:
+        raise NotImplementedError("batch_optimize: saving optimisers is not provided; pass save=False")
+    if not isinstance(optimizers, list) and not isinstance(problems, list):
+        raise ValueError("At least one of optimizers or problems must be a list")
+    if not isinstance(optimizers, list):
+        optimizers = [optimizers] * len(problems)
+    if not isinstance(problems, list):
+        problems = [problems] * len(optimizers)
+    if len(optimizers) != len(problems):
+        raise ValueError("Same number of optimizers and problems must be provided.")
+    if not all(isinstance(o, CMAESOptimizer) for o in optimizers):
+        raise TypeError("Batch optimization is only supported for population-based optimizers (CMAESOptimizer)")
+    if len({type(p.sim_group) for p in problems}) != 1:
+        raise ValueError("All optimizations must be done on the same type of model.")
+    optimizers = [copy.deepcopy(o) for o in optimizers]
+    problems = [copy.deepcopy(p) for p in problems]
+    for o, p in zip(optimizers, problems):
+        o.setup_problem(p)
+        o._start()
+    while any(o._has_next() for o in optimizers):
+        ongoing = [o for o in optimizers if o._has_next()]
+        Xs = []
+        for o in ongoing:
+            X = o._ask()
+            o.problem.sim_group.N = X.shape[0]
+            o.problem._set_sim_params(X)
+            Xs.append(X)
+        sim.MultiSimGroup([o.problem.sim_group for o in ongoing]).run()
+        for o, X in zip(ongoing, Xs):
+            out, scores = {}, []
+            o.problem._evaluate(X, out, scores=scores, skip_run=True)
+            o._tell(X, out["F"], scores[0])
+    for o in optimizers:
+        o._finish()
+    return optimizers

@@ -5,6 +5,8 @@
   same noise: identical scores) and its costs are -(+gof).
 * CMAESOptimizer (popsize 4, 2 generations after the uniform first one)
   keeps every candidate inside the bounds and reports the best row.
+* batch_optimize of one CMA-ES over two problems with different SCs: one
+  simulation call per generation, histories equal to independent runs.
 The arithmetic of both against the oracle is in tests/test_optimize.py.
 """
 import numpy as np
@@ -21,8 +23,9 @@ def _problem(**kw):
     import torch  # noqa: F401
     from cubnm_amd.optimize import BNMProblem
     rng = np.random.default_rng(0)
+    sc = kw.pop("sc", cases.synthetic_sc(N, seed=1))
     return BNMProblem(model="rWW", params={"G": (0.5, 3.0), "w_p": (0.5, 1.5), "J_N": 0.15}, duration=12, TR=1,
-                      sc=cases.synthetic_sc(N, seed=1), window_size=4, window_step=2, bold_remove_s=2,
+                      sc=sc, window_size=4, window_step=2, bold_remove_s=2,
                       do_fic=False, emp_fc_tril=np.tanh(rng.standard_normal(N * (N - 1) // 2)),
                       emp_fcd_tril=np.tanh(rng.standard_normal(6)), **kw)
 
@@ -55,3 +58,23 @@ def test_cmaes_optimizer_runs():
     assert len(h) == 12
     assert ((h["G"] >= 0.5) & (h["G"] <= 3.0)).all() and ((h["w_p"] >= 0.5) & (h["w_p"] <= 1.5)).all()
     assert opt["cost"] == h["cost"].min()
+
+
+def test_batch_optimize_variable_sc_matches_independent_runs():
+    """Reference tests/optimize/test.py:157-193 on the GPU core: one CMA-ES
+    on two problems with different SCs (the second reversed) in one call per
+    generation; each history equals the independent run to 1e-6 (the
+    reference's tolerance), and the two differ."""
+    from cubnm_amd.optimize import CMAESOptimizer, batch_optimize
+    p1 = _problem()
+    p2 = _problem(sc=cases.synthetic_sc(N, seed=1)[::-1, ::-1].copy())
+    o = CMAESOptimizer(popsize=4, n_iter=2, seed=1)
+    b = batch_optimize(o, [p1, p2])
+    assert not np.allclose(b[0].history["cost"].values, b[1].history["cost"].values, atol=1e-6)
+    for k, p in enumerate((p1, p2)):
+        alone = CMAESOptimizer(popsize=4, n_iter=2, seed=1)
+        alone.setup_problem(p)
+        alone.optimize()
+        assert b[k].history.shape == alone.history.shape
+        np.testing.assert_allclose(b[k].history.values.astype(float), alone.history.values.astype(float),
+                                   rtol=0, atol=1e-6)

@@ -18,6 +18,10 @@ what it must satisfy rather than against a restated formula:
 With run_simulations swapped for the oracle (test infrastructure), the grid
 and CMA-ES drivers evaluate the problem end to end: costs are -(+gof), the
 FIC penalty enters the cost, and the grid's rows equal a direct SimGroup run.
+batch_optimize (one simulation call per generation for several optimisers)
+gives exactly the histories of the same optimisers run alone, on one problem
+and on two problems with different SCs, and raises the reference's argument
+errors.
 """
 import numpy as np
 import pytest
@@ -183,3 +187,62 @@ def test_cmaes_optimizer_end_to_end(oracle_core):
     assert len(h) == 12 and sorted(h["gen"].unique().tolist()) == [0, 1, 2]
     assert ((h["G"] >= 0.5) & (h["G"] <= 3.0) & (h["w_p"] >= 0.5) & (h["w_p"] <= 1.5)).all()
     assert opt["cost"] == h["cost"].min() and len(cm.opt_history) == 3
+
+
+def _history_equal(a, b):
+    return a.shape == b.shape and np.array_equal(a.values, b.values)
+
+
+def test_batch_optimize_matches_independent_runs(oracle_core):
+    """Reference tests/optimize/test.py:119-155: four optimisers (two seeds,
+    each twice) on one problem in one batch; equal seeds give equal
+    histories, different seeds different ones, and each equals the same
+    optimiser run alone."""
+    from cubnm_amd.optimize import CMAESOptimizer, batch_optimize
+    p = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5)})
+    o1, o2 = CMAESOptimizer(popsize=4, n_iter=2, seed=1), CMAESOptimizer(popsize=4, n_iter=2, seed=2)
+    b = batch_optimize([o1, o1, o2, o2], p)
+    assert len(b) == 4 and all(o.is_fit for o in b)
+    assert _history_equal(b[0].history, b[1].history) and _history_equal(b[2].history, b[3].history)
+    assert not np.allclose(b[0].history[["G", "w_p"]].values, b[2].history[["G", "w_p"]].values)
+    import copy
+    for o, k in ((o1, 0), (o2, 2)):
+        o.setup_problem(copy.deepcopy(p))
+        o.optimize()
+        assert _history_equal(b[k].history, o.history)
+        assert b[k].opt["cost"] == o.opt["cost"]
+
+
+def test_batch_optimize_variable_sc(oracle_core):
+    """Reference tests/optimize/test.py:157-193: one optimiser on two
+    problems that differ in SC (the second reversed) runs as one call with
+    two SCs; each history equals the independent run on its problem."""
+    from cubnm_amd.optimize import CMAESOptimizer, batch_optimize
+    p1 = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5)})
+    p2 = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5)}, sc=cases.synthetic_sc(N, seed=1)[::-1, ::-1].copy())
+    o = CMAESOptimizer(popsize=4, n_iter=2, seed=1)
+    b = batch_optimize(o, [p1, p2])
+    assert not np.allclose(b[0].history["cost"].values, b[1].history["cost"].values)
+    for k, p in enumerate((p1, p2)):
+        alone = CMAESOptimizer(popsize=4, n_iter=2, seed=1)
+        alone.setup_problem(p)
+        alone.optimize()
+        assert _history_equal(b[k].history, alone.history)
+
+
+def test_batch_optimize_argument_errors():
+    from cubnm_amd.optimize import BNMProblem, CMAESOptimizer, GridOptimizer, batch_optimize
+    p = _problem({"G": (0.5, 3.0)})
+    o = CMAESOptimizer(popsize=4, n_iter=1)
+    with pytest.raises(ValueError, match="must be a list"):
+        batch_optimize(o, p)
+    with pytest.raises(ValueError, match="Same number"):
+        batch_optimize([o, o], [p, p, p])
+    with pytest.raises(TypeError, match="population-based"):
+        batch_optimize([GridOptimizer()], [p])
+    with pytest.raises(ValueError, match="same type of model"):
+        batch_optimize(o, [p, BNMProblem(model="JR", params={"G": (0.5, 3.0)}, duration=12, TR=1,
+                                         sc=cases.synthetic_sc(N, seed=1), window_size=4, window_step=2,
+                                         bold_remove_s=2, emp_fc_tril=p.emp_fc_tril, emp_fcd_tril=p.emp_fcd_tril)])
+    with pytest.raises(NotImplementedError):
+        batch_optimize([o], p, save=True)
