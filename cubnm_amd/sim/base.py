@@ -97,6 +97,7 @@ class SimGroup:
         self.fcd_drop_edges = fcd_drop_edges
         self.noise_segment_length = noise_segment_length
         self.sim_verbose, self.progress_interval = sim_verbose, progress_interval
+        self.avail_gpus, self.avail_cpus = utils.avail_gpus(), os.cpu_count()   # base.py:246-248
         self.bw_params = bw_params
         self.out_dir = self.input_out_dir = out_dir
         self.param_lists = {k: None for k in self.global_param_names + self.regional_param_names + ["v"]}

@@ -7,14 +7,47 @@ path's caller side:
   calculate_fcd   utils.py:204-304  (empirical FCD tril)
   fc_corr_device  utils.py:416-507  (Pearson r of each simulated FC vs empirical)
   fcd_ks_device   utils.py:306-414  (KS distance of each simulated FCD vs empirical)
+  avail_gpus      utils.py:68-111   (GPUs on the machine)
+  is_jupyter      utils.py:113-128  (running under a Jupyter kernel)
 The two *_device functions run on the GPU through the C ABI
 (cubnm_gof_fc_corr / cubnm_gof_fcd_ks); the empirical helpers are the
 reference's NumPy restated (they run once per problem).
 """
+import glob
+import sys
+
 import numpy as np
 import scipy.stats
 
 from . import _core
+
+
+_KFD_TOPOLOGY = "/sys/class/kfd/kfd/topology/nodes"
+
+
+def avail_gpus():
+    """Number of GPUs on the machine, counted from the kernel driver's
+    topology (nodes with SIMDs are GPUs; CPU nodes report 0), without
+    initialising a GPU runtime.  0 where the driver is absent."""
+    n = 0
+    for props in glob.glob(f"{_KFD_TOPOLOGY}/*/properties"):
+        try:
+            with open(props) as f:
+                for line in f:
+                    key, _, val = line.partition(" ")
+                    if key == "simd_count" and int(val) > 0:
+                        n += 1
+                        break
+        except (OSError, ValueError):
+            continue
+    return n
+
+
+def is_jupyter():
+    """True when running under a Jupyter (IPython kernel) session."""
+    ipython = sys.modules.get("IPython")
+    shell = ipython.get_ipython() if ipython is not None and hasattr(ipython, "get_ipython") else None
+    return shell is not None and "IPKernelApp" in getattr(shell, "config", {})
 
 
 def get_bw_params(src):

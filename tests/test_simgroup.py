@@ -196,3 +196,20 @@ def test_multi_sim_group_split(oracle_core):
         g.run()
         np.testing.assert_array_equal(g.sim_bold, bold)
         np.testing.assert_array_equal(g.sim_fc_trils, fc)
+
+
+def test_avail_gpus_counts_gpu_topology_nodes(tmp_path, monkeypatch):
+    """utils.avail_gpus counts kernel-driver topology nodes with SIMDs: a CPU
+    node (simd_count 0) and an unreadable entry are not GPUs."""
+    from cubnm_amd import utils
+    for i, simds in enumerate([0, 1024, 1024]):
+        d = tmp_path / str(i)
+        d.mkdir()
+        (d / "properties").write_text(f"cpu_cores_count {0 if simds else 64}\nsimd_count {simds}\nmax_waves_per_simd 8\n")
+    (tmp_path / "3").mkdir()
+    (tmp_path / "3" / "properties").write_text("simd_count x\n")
+    monkeypatch.setattr(utils, "_KFD_TOPOLOGY", str(tmp_path))
+    assert utils.avail_gpus() == 2
+    monkeypatch.setattr(utils, "_KFD_TOPOLOGY", str(tmp_path / "absent"))
+    assert utils.avail_gpus() == 0
+    assert utils.is_jupyter() is False
