@@ -11,11 +11,12 @@ reference's ``cubnm.sim`` classes runs unchanged:
   get_noise                         sim/base.py:845-887
   get_sim_fc / get_sim_fcd          sim/base.py:889-960
   score (GOF terms, NaN -> worst)   sim/base.py:37-42, 1084-1207
+  save (outputs and parameters)     sim/base.py:1209-1239
   multi-group merge and split       sim/base.py:1303-1438
 
 Differences: the core has no CPU engine, so ``force_cpu=True`` only affects
 ``score`` (SciPy instead of the GPU reductions) and ``run`` raises; nodes are
-limited to 1024 (no co-launch); saving to ``out_dir`` is not implemented.
+limited to 1024 (no co-launch).
 """
 import copy
 import gc
@@ -105,6 +106,11 @@ class SimGroup:
         self.last_N = 0
         self.last_config = self.get_config(for_reinit=True)
         self.it = 0
+        self.post_init()
+
+    def post_init(self):
+        """Hook run at the end of construction; subclasses may extend it
+        (base.py:286-294)."""
 
     # ---- timing -----------------------------------------------------------
     @property
@@ -369,6 +375,27 @@ class SimGroup:
                 self.sim_fcd_trils = self.sim_fcd_trils.reshape(self.N, -1)
 
     # ---- outputs -------------------------------------------------------------
+    def _get_save_data(self):
+        """Outputs and parameters written by ``save`` (base.py:1209-1227):
+        sim_bold, one array per state variable, the FC / FCD trils when
+        computed, and every parameter list."""
+        data = {"sim_bold": self.sim_bold}
+        data.update(self.sim_states)
+        if self.do_fc:
+            data["sim_fc_trils"] = self.sim_fc_trils
+            if self.do_fcd:
+                data["sim_fcd_trils"] = self.sim_fcd_trils
+        data.update({k: v for k, v in self.param_lists.items() if v is not None})
+        return data
+
+    def save(self):
+        """Write the outputs of the last run to ``out_dir/sim_data.npz``
+        (compressed; base.py:1229-1239)."""
+        if self.out_dir is None:
+            raise ValueError("Cannot save the simulations when `.out_dir` is not set")
+        os.makedirs(self.out_dir, exist_ok=True)
+        np.savez_compressed(os.path.join(self.out_dir, "sim_data.npz"), **self._get_save_data())
+
     def get_state_averages(self):
         import pandas as pd
         avg = {}

@@ -12,7 +12,9 @@ multi-group merge are checked without a GPU:
   * score(): -fc_diff / -fc_normec / +gof arithmetic, SciPy terms, NaN ->
     worst score, FIC penalty (rww.py:180-234);
   * MultiSimGroup merges duplicate SCs and splits outputs so each child sees
-    what it would see alone.
+    what it would see alone;
+  * save() writes the outputs and parameter lists (base.py:1209-1239) and
+    the file reads back with numpy's non-pickle loader.
 """
 from decimal import Decimal
 
@@ -213,3 +215,25 @@ def test_avail_gpus_counts_gpu_topology_nodes(tmp_path, monkeypatch):
     monkeypatch.setattr(utils, "_KFD_TOPOLOGY", str(tmp_path / "absent"))
     assert utils.avail_gpus() == 0
     assert utils.is_jupyter() is False
+
+
+def test_save_writes_outputs_and_parameters(oracle_core, tmp_path):
+    from cubnm_amd.sim import rWWSimGroup
+    sg = rWWSimGroup(duration=12, TR=1, sc=cases.synthetic_sc(10), window_size=4, window_step=2,
+                     bold_remove_s=2, out_dir=str(tmp_path / "sims"))
+    sg.N = 2
+    sg.param_lists["G"] = np.array([0.5, 1.5])
+    sg._set_default_params()
+    sg.run()
+    sg.save()
+    with np.load(tmp_path / "sims" / "sim_data.npz") as d:
+        expected = {"sim_bold", "sim_fc_trils", "sim_fcd_trils", *sg.state_names,
+                    *[k for k, v in sg.param_lists.items() if v is not None]}
+        assert set(d.files) == expected
+        np.testing.assert_array_equal(d["sim_bold"], sg.sim_bold)
+        np.testing.assert_array_equal(d["sim_fcd_trils"], sg.sim_fcd_trils)
+        np.testing.assert_array_equal(d["wIE"], sg.param_lists["wIE"])     # after the FIC write-back
+        np.testing.assert_array_equal(d["S_E"], sg.sim_states["S_E"])
+    sg.out_dir = None
+    with pytest.raises(ValueError, match="out_dir"):
+        sg.save()
