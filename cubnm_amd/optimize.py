@@ -280,9 +280,97 @@ class BNMProblem:
         return out["F"], scores[0]
 
 
-class GridOptimizer:
+def _json_value(v):
+    if isinstance(v, np.generic):
+        return v.item()
+    if isinstance(v, (set, tuple)):
+        return list(v)
+    return str(v)
+
+
+class _Saving:
+    """``save`` of the reference's Optimizer (optimize.py:636-731): a new
+    ``<name>_run-<i>`` directory under the problem's ``out_dir`` with
+    history.csv, opt.csv and opt_history.csv (single objective),
+    problem.json (arrays as .txt files beside it, existing paths made
+    absolute), optimizer.json and, with ``save_opt``, the optimal
+    simulation(s) in opt_sim/ (sim_data.npz; scores.csv when rerun).
+    ``save_obj`` (a pickle of the whole optimiser) is not provided, and the
+    problem's SimGroup keeps its own ``out_dir`` after the optimum is
+    written (the reference leaves it pointing at opt_sim/)."""
+
+    def get_config(self):
+        return {"n_iter": self.n_iter, "popsize": self.popsize, "seed": self.seed}
+
+    def save(self, save_opt=True, save_obj=False):
+        import json
+        if not getattr(self, "is_fit", False):
+            raise RuntimeError("Optimizer must be fitted before saving. Call `optimize()` method to fit the optimizer.")
+        if save_obj:
+            raise NotImplementedError("save_obj: pickling the optimizer is not provided")
+        out_dir = self.problem.sim_group.out_dir
+        if out_dir is None:
+            raise ValueError("Cannot save the optimizer when the problem's `out_dir` is not set")
+        name = type(self).__name__.lower().replace("optimizer", "")
+        run_idx = 0
+        while os.path.exists(os.path.join(out_dir, f"{name}_run-{run_idx}")):
+            run_idx += 1
+        opt_dir = os.path.join(out_dir, f"{name}_run-{run_idx}")
+        os.makedirs(opt_dir)
+        self.history.to_csv(os.path.join(opt_dir, "history.csv"))
+        if self.problem.n_obj == 1:
+            self.opt.to_csv(os.path.join(opt_dir, "opt.csv"))
+            if not isinstance(self, GridOptimizer):
+                self.opt_history.to_csv(os.path.join(opt_dir, "opt_history.csv"))
+        config = self.problem.get_config(include_sim_group=True, include_N=True)
+        for k, v in config.items():
+            if isinstance(v, np.ndarray):
+                path = os.path.join(opt_dir, k + ".txt")
+                np.savetxt(path, v.reshape(v.shape[0], -1) if v.ndim > 2 else v)
+                config[k] = path
+            if isinstance(config[k], (str, os.PathLike)) and os.path.exists(config[k]):
+                config[k] = os.path.abspath(config[k])
+        with open(os.path.join(opt_dir, "problem.json"), "w") as f:
+            json.dump(config, f, indent=4, default=_json_value)
+        with open(os.path.join(opt_dir, "optimizer.json"), "w") as f:
+            json.dump(self.get_config(), f, indent=4, default=_json_value)
+        if save_opt:
+            opt_sim_dir = os.path.join(opt_dir, "opt_sim")
+            os.makedirs(opt_sim_dir, exist_ok=True)
+            sg = self._opt_sim_group(opt_sim_dir)
+            prev, sg.out_dir = sg.out_dir, opt_sim_dir
+            sg.save()
+            if sg is self.problem.sim_group:   # the problem keeps saving under its own out_dir
+                sg.out_dir = prev
+            self.problem.opt_sim_group = sg
+        return opt_dir
+
+    def _opt_sim_group(self, opt_sim_dir):
+        """Rerun the best candidate and write its scores (optimize.py:717-727)."""
+        X = self.problem._get_X(np.atleast_2d(self.opt[self.problem.free_params].values.astype(float)))
+        Xt = pd.DataFrame(self.problem._get_Xt(X), columns=self.problem.free_params)
+        pd.concat([Xt, self.problem.eval(X)], axis=1).to_csv(os.path.join(opt_sim_dir, "scores.csv"))
+        return self.problem.sim_group
+
+
+class GridOptimizer(_Saving):
     """Every point of a grid over the free parameters, evaluated in one
     batch (optimize.py:1168-1233)."""
+
+    def get_config(self):
+        return {"grid_shape": self.grid_shape}
+
+    def save(self, save_avg_states=True, **kwargs):
+        """As the base ``save``, plus state_averages.csv of every grid point
+        (optimize.py:1235-1254); the optimal simulation is sliced out of the
+        grid run instead of rerun."""
+        opt_dir = super().save(**kwargs)
+        if save_avg_states:
+            self.problem.sim_group.get_state_averages().to_csv(os.path.join(opt_dir, "state_averages.csv"))
+        return opt_dir
+
+    def _opt_sim_group(self, opt_sim_dir):
+        return self.problem.sim_group.slice(int(self.opt.name), inplace=False)
 
     def optimize(self, problem, grid_shape):
         self.problem = problem
@@ -303,7 +391,7 @@ class GridOptimizer:
         return self.opt
 
 
-class CMAESOptimizer:
+class CMAESOptimizer(_Saving):
     """CMA-ES in the unit cube over a BNMProblem's free parameters
     (optimize.py:1386-1486): with x0=None the first generation is popsize
     uniform samples whose best point becomes the initial mean; sigma 0.5;
@@ -378,7 +466,7 @@ class CMAESOptimizer:
         return self._finish()
 
 
-def batch_optimize(optimizers, problems, save=False):
+def batch_optimize(optimizers, problems, save=True):
     """Run several CMA-ES optimisations together (optimize.py:1506-1715):
     each generation, the candidates of every optimisation still running are
     simulated in one call through a MultiSimGroup (SCs merged, duplicates
@@ -388,11 +476,10 @@ def batch_optimize(optimizers, problems, save=False):
     are deep-copied, so the same object may appear several times; every
     copy draws from its own seeded generators, so each history equals that
     of the same optimiser run alone on its problem.  Returns the list of
-    fitted optimisers.  ``save=True`` (the reference's default, which writes
-    the optima to disk) is not provided and raises NotImplementedError."""
+    fitted optimisers.  With ``save`` (the default) each optimiser is saved
+    as by its ``save()`` (the optima rerun one problem at a time), which
+    needs the problems' ``out_dir``."""
     import copy
-    if save:
-        raise NotImplementedError("batch_optimize: saving optimisers is not provided; pass save=False")
     if not isinstance(optimizers, list) and not isinstance(problems, list):
         raise ValueError("At least one of optimizers or problems must be a list")
     if not isinstance(optimizers, list):
@@ -405,6 +492,8 @@ def batch_optimize(optimizers, problems, save=False):
         raise TypeError("Batch optimization is only supported for population-based optimizers (CMAESOptimizer)")
     if len({type(p.sim_group) for p in problems}) != 1:
         raise ValueError("All optimizations must be done on the same type of model.")
+    if save and any(p.sim_group.out_dir is None for p in problems):
+        raise ValueError("save=True needs every problem's `out_dir` (or pass save=False)")
     optimizers = [copy.deepcopy(o) for o in optimizers]
     problems = [copy.deepcopy(p) for p in problems]
     for o, p in zip(optimizers, problems):
@@ -425,4 +514,7 @@ def batch_optimize(optimizers, problems, save=False):
             o._tell(X, out["F"], scores[0])
     for o in optimizers:
         o._finish()
+    if save:
+        for o in optimizers:
+            o.save()
     return optimizers

@@ -69,7 +69,7 @@ def test_batch_optimize_variable_sc_matches_independent_runs():
     p1 = _problem()
     p2 = _problem(sc=cases.synthetic_sc(N, seed=1)[::-1, ::-1].copy())
     o = CMAESOptimizer(popsize=4, n_iter=2, seed=1)
-    b = batch_optimize(o, [p1, p2])
+    b = batch_optimize(o, [p1, p2], save=False)
     assert not np.allclose(b[0].history["cost"].values, b[1].history["cost"].values, atol=1e-6)
     for k, p in enumerate((p1, p2)):
         alone = CMAESOptimizer(popsize=4, n_iter=2, seed=1)

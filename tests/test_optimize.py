@@ -23,6 +23,8 @@ gives exactly the histories of the same optimisers run alone, on one problem
 and on two problems with different SCs, and raises the reference's argument
 errors.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -201,7 +203,7 @@ def test_batch_optimize_matches_independent_runs(oracle_core):
     from cubnm_amd.optimize import CMAESOptimizer, batch_optimize
     p = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5)})
     o1, o2 = CMAESOptimizer(popsize=4, n_iter=2, seed=1), CMAESOptimizer(popsize=4, n_iter=2, seed=2)
-    b = batch_optimize([o1, o1, o2, o2], p)
+    b = batch_optimize([o1, o1, o2, o2], p, save=False)
     assert len(b) == 4 and all(o.is_fit for o in b)
     assert _history_equal(b[0].history, b[1].history) and _history_equal(b[2].history, b[3].history)
     assert not np.allclose(b[0].history[["G", "w_p"]].values, b[2].history[["G", "w_p"]].values)
@@ -221,7 +223,7 @@ def test_batch_optimize_variable_sc(oracle_core):
     p1 = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5)})
     p2 = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5)}, sc=cases.synthetic_sc(N, seed=1)[::-1, ::-1].copy())
     o = CMAESOptimizer(popsize=4, n_iter=2, seed=1)
-    b = batch_optimize(o, [p1, p2])
+    b = batch_optimize(o, [p1, p2], save=False)
     assert not np.allclose(b[0].history["cost"].values, b[1].history["cost"].values)
     for k, p in enumerate((p1, p2)):
         alone = CMAESOptimizer(popsize=4, n_iter=2, seed=1)
@@ -244,5 +246,44 @@ def test_batch_optimize_argument_errors():
         batch_optimize(o, [p, BNMProblem(model="JR", params={"G": (0.5, 3.0)}, duration=12, TR=1,
                                          sc=cases.synthetic_sc(N, seed=1), window_size=4, window_step=2,
                                          bold_remove_s=2, emp_fc_tril=p.emp_fc_tril, emp_fcd_tril=p.emp_fcd_tril)])
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError, match="out_dir"):
         batch_optimize([o], p, save=True)
+
+
+def test_optimizer_save_layout(oracle_core, tmp_path):
+    """Optimizer.save (optimize.py:636-731, 1235-1254): run directories
+    numbered in turn, the history tables, problem.json with arrays moved to
+    .txt files, optimizer.json, and the optimum's simulation in opt_sim/
+    (sliced out of the grid run; rerun for CMA-ES, whose scores.csv row
+    equals the best history row)."""
+    import json
+    import pandas as pd
+    from cubnm_amd.optimize import CMAESOptimizer, GridOptimizer, batch_optimize
+    p = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5)}, out_dir=str(tmp_path))
+    go = GridOptimizer()
+    go.optimize(p, grid_shape=2)
+    d = go.save()
+    assert d == str(tmp_path / "grid_run-0")
+    assert {"history.csv", "opt.csv", "problem.json", "optimizer.json", "opt_sim", "state_averages.csv",
+            "sc.txt"} <= set(os.listdir(d))
+    assert "opt_history.csv" not in os.listdir(d)
+    cfg = json.load(open(os.path.join(d, "problem.json")))
+    np.testing.assert_array_equal(np.loadtxt(cfg["sc"]), p.sim_group.sc)
+    assert json.load(open(os.path.join(d, "optimizer.json"))) == {"grid_shape": {"G": 2, "w_p": 2}}
+    with np.load(os.path.join(d, "opt_sim", "sim_data.npz")) as z:
+        k = int(go.opt.name)
+        np.testing.assert_array_equal(z["G"], [go.history["G"][k]])
+        assert z["sim_bold"].shape[0] == 1
+    assert go.save(save_opt=False) == str(tmp_path / "grid_run-1")
+    cm = CMAESOptimizer(popsize=4, n_iter=1, seed=2)
+    cm.setup_problem(p)
+    cm.optimize()
+    d = cm.save()
+    assert d == str(tmp_path / "cmaes_run-0") and "opt_history.csv" in os.listdir(d)
+    assert json.load(open(os.path.join(d, "optimizer.json"))) == {"n_iter": 1, "popsize": 4, "seed": 2}
+    sc = pd.read_csv(os.path.join(d, "opt_sim", "scores.csv"))
+    np.testing.assert_allclose(sc["+gof"].values, [cm.opt["+gof"]], rtol=0, atol=1e-12)
+    with pytest.raises(RuntimeError, match="fitted"):
+        CMAESOptimizer(popsize=4).save()
+    b = batch_optimize(cm, [p, p])
+    assert os.path.isdir(tmp_path / "cmaes_run-1") and os.path.isdir(tmp_path / "cmaes_run-2") and len(b) == 2
