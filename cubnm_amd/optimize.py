@@ -14,8 +14,11 @@ rWW FIC penalty hook.  ``GridOptimizer`` (optimize.py:1168-1233) and
 ``CMAESOptimizer`` (optimize.py:1386-1486; here driven by this package's
 own CMA-ES, ``fit.CMAES``, in the unit cube) produce ``history``,
 ``opt_history`` and ``opt`` tables with the reference's columns.
-``batch_optimize`` (optimize.py:1506-1715) runs several CMA-ES optimisations
-with one simulation call per generation.
+``NSGA2Optimizer`` (optimize.py:1487-1504; this package's NSGA-II with
+pymoo's default operators) handles up to three objectives, and ``save``
+(optimize.py:636-731) writes the reference's run directory.
+``batch_optimize`` (optimize.py:1506-1715) runs several CMA-ES or NSGA-II
+optimisations with one simulation call per generation.
 """
 import itertools
 import logging
@@ -345,9 +348,14 @@ class _Saving:
             self.problem.opt_sim_group = sg
         return opt_dir
 
+    def _opt_X(self):
+        return self.problem._get_X(np.atleast_2d(self.opt[self.problem.free_params].values.astype(float)))
+
     def _opt_sim_group(self, opt_sim_dir):
-        """Rerun the best candidate and write its scores (optimize.py:717-727)."""
-        X = self.problem._get_X(np.atleast_2d(self.opt[self.problem.free_params].values.astype(float)))
+        """Rerun the optimum (the best candidate, or the final population's
+        non-dominated set with several objectives) and write its scores
+        (optimize.py:717-727)."""
+        X = self._opt_X()
         Xt = pd.DataFrame(self.problem._get_Xt(X), columns=self.problem.free_params)
         pd.concat([Xt, self.problem.eval(X)], axis=1).to_csv(os.path.join(opt_sim_dir, "scores.csv"))
         return self.problem.sim_group
@@ -391,71 +399,59 @@ class GridOptimizer(_Saving):
         return self.opt
 
 
-class CMAESOptimizer(_Saving):
-    """CMA-ES in the unit cube over a BNMProblem's free parameters
-    (optimize.py:1386-1486): with x0=None the first generation is popsize
-    uniform samples whose best point becomes the initial mean; sigma 0.5;
-    popsize candidates per generation for n_iter generations after it.
-    Uses this package's CMA-ES (fit.CMAES, Hansen's tutorial defaults,
-    samples clipped to the cube) instead of pymoo's wrapper of `cma`, so the
-    trajectories are not the reference's."""
+class _PopulationOptimizer(_Saving):
+    """Generation-at-a-time driver shared by the population optimisers, so
+    that batch_optimize can run the generations of several optimisers as
+    one simulation call.  A subclass provides _start_algorithm, _ask and
+    _tell_algorithm; history rows are the candidates' parameters, their
+    objective values (column 'cost' for one objective, the problem's
+    obj_names otherwise), with one objective the score columns, and 'gen'."""
 
-    def __init__(self, popsize, n_iter=2, seed=0, sigma=0.5, x0=None, print_history=False):
-        self.popsize, self.n_iter, self.seed, self.sigma, self.x0 = popsize, n_iter, seed, sigma, x0
+    def __init__(self, popsize, n_iter=2, seed=0, print_history=False):
+        self.popsize, self.n_iter, self.seed = popsize, n_iter, seed
         self.print_history = print_history
 
     def setup_problem(self, problem):
-        if problem.n_obj != 1:
-            raise ValueError("CMA-ES optimises a single objective")
+        if problem.n_obj > self.max_obj:
+            raise ValueError(f"{type(self).__name__} optimises at most {self.max_obj} objective(s)")
         self.problem = problem
 
     def _record(self, X, F, scores, gen):
         Xt = pd.DataFrame(self.problem._get_Xt(X), columns=self.problem.free_params)
         Xt.index.name = "sim_idx"
-        res = pd.concat([Xt, pd.DataFrame({"cost": F}), scores], axis=1)
+        single = self.problem.n_obj == 1
+        Fd = pd.DataFrame(np.reshape(F, (len(X), -1)), columns=["cost"] if single else self.problem.obj_names)
+        res = pd.concat([Xt, Fd, scores] if single else [Xt, Fd], axis=1)
         res["gen"] = gen
         if self.print_history:
             print(res.to_string())
         self.history.append(res)
-        self.opt_history.append(res.loc[res["cost"].argmin()])
+        if single:
+            self.opt_history.append(res.loc[res["cost"].argmin()])
 
-    # One generation at a time, so that batch_optimize can run the
-    # generations of several optimisers as one simulation call.
     def _start(self):
-        from .fit import CMAES
         self._rng = np.random.default_rng(self.seed)
         self.history, self.opt_history = [], []
-        self._es = None
         self._gen = 0
-        if self.x0 is not None:
-            x0 = self.problem._get_X(np.asarray(self.x0, float))
-            self._es = CMAES(x0, self.sigma, self.popsize, seed=self.seed + 1)
-            self._gen = 1
+        self._start_algorithm()
 
     def _has_next(self):
         return self._gen <= self.n_iter
 
-    def _ask(self):
-        if self._es is None:      # generation 0: uniform samples in the cube
-            return self._rng.uniform(0, 1, (self.popsize, self.problem.ndim))
-        return self._es.ask()
-
     def _tell(self, X, F, scores):
-        from .fit import CMAES
         F = np.where(np.isnan(F), np.inf, F)
-        if self._es is None:
-            self._es = CMAES(X[int(np.argmin(F))], self.sigma, self.popsize, seed=self.seed + 1)
-        else:
-            self._es.tell(X, F)
+        self._tell_algorithm(X, F)
         self._record(X, F, scores, self._gen)
         self._gen += 1
 
     def _finish(self):
         self.history = pd.concat(self.history, axis=0).reset_index(drop=False)
-        self.opt_history = pd.DataFrame(self.opt_history).reset_index(drop=True)
-        self.opt = self.history.loc[self.history["cost"].argmin()]
         self.is_fit = True
-        return self.opt
+        if self.problem.n_obj == 1:
+            self.opt_history = pd.DataFrame(self.opt_history).reset_index(drop=True)
+            self.opt = self.history.loc[self.history["cost"].argmin()]
+            return self.opt
+        return None
 
     def optimize(self):
         self._start()
@@ -466,8 +462,194 @@ class CMAESOptimizer(_Saving):
         return self._finish()
 
 
+class CMAESOptimizer(_PopulationOptimizer):
+    """CMA-ES in the unit cube over a BNMProblem's free parameters
+    (optimize.py:1386-1486): with x0=None the first generation is popsize
+    uniform samples whose best point becomes the initial mean; sigma 0.5;
+    popsize candidates per generation for n_iter generations after it.
+    Uses this package's CMA-ES (fit.CMAES, Hansen's tutorial defaults,
+    samples clipped to the cube) instead of pymoo's wrapper of `cma`, so the
+    trajectories are not the reference's."""
+    max_obj = 1
+
+    def __init__(self, popsize, n_iter=2, seed=0, sigma=0.5, x0=None, print_history=False):
+        super().__init__(popsize, n_iter, seed, print_history)
+        self.sigma, self.x0 = sigma, x0
+
+    def setup_problem(self, problem):
+        if problem.n_obj != 1:
+            raise ValueError("CMA-ES optimises a single objective")
+        self.problem = problem
+
+    def _start_algorithm(self):
+        from .fit import CMAES
+        self._es = None
+        if self.x0 is not None:
+            x0 = self.problem._get_X(np.asarray(self.x0, float))
+            self._es = CMAES(x0, self.sigma, self.popsize, seed=self.seed + 1)
+            self._gen = 1
+
+    def _ask(self):
+        if self._es is None:      # generation 0: uniform samples in the cube
+            return self._rng.uniform(0, 1, (self.popsize, self.problem.ndim))
+        return self._es.ask()
+
+    def _tell_algorithm(self, X, F):
+        from .fit import CMAES
+        if self._es is None:
+            self._es = CMAES(X[int(np.argmin(F))], self.sigma, self.popsize, seed=self.seed + 1)
+        else:
+            self._es.tell(X, F)
+
+
+def _nondominated_ranks(F):
+    """Front index of every row of F (minimised objectives): front 0 is the
+    rows no other row dominates, front r+1 those dominated only by rows of
+    fronts <= r (Deb et al. 2002, fast non-dominated sorting)."""
+    F = np.asarray(F, float).reshape(len(F), -1)
+    le = np.all(F[:, None, :] <= F[None, :, :], axis=2)
+    lt = np.any(F[:, None, :] < F[None, :, :], axis=2)
+    dom = le & lt                                  # dom[i, j]: i dominates j
+    count = dom.sum(axis=0)
+    rank = np.full(len(F), -1)
+    left = np.ones(len(F), bool)
+    r = 0
+    while left.any():
+        front = np.where(left & (count == 0))[0]
+        rank[front] = r
+        left[front] = False
+        count = count - dom[front].sum(axis=0)
+        r += 1
+    return rank
+
+
+def _crowding_distance(F):
+    """Crowding distance of the rows of one front: per objective, the gap
+    between the two neighbours normalised by the front's range, summed;
+    the extreme rows of every objective are infinite."""
+    F = np.asarray(F, float).reshape(len(F), -1)
+    n = len(F)
+    if n <= 2:
+        return np.full(n, np.inf)
+    d = np.zeros(n)
+    for k in range(F.shape[1]):
+        o = np.argsort(F[:, k], kind="stable")
+        f = F[o, k]
+        d[o[0]] = d[o[-1]] = np.inf
+        span = f[-1] - f[0]
+        if span > 0 and np.isfinite(span):
+            d[o[1:-1]] += (f[2:] - f[:-2]) / span
+    return d
+
+
+def _rank_and_crowding(F):
+    rank = _nondominated_ranks(F)
+    crowd = np.empty(len(rank))
+    for r in np.unique(rank):
+        m = rank == r
+        crowd[m] = _crowding_distance(np.asarray(F).reshape(len(rank), -1)[m])
+    return rank, crowd
+
+
+class NSGA2Optimizer(_PopulationOptimizer):
+    """NSGA-II (Deb et al. 2002) in the unit cube over a BNMProblem's free
+    parameters, for up to three objectives (optimize.py:1487-1504):
+    generation 0 is popsize uniform samples; each later generation makes
+    popsize offspring by binary tournaments on (front, crowding distance),
+    simulated binary crossover (eta 15, probability 0.9 per pair, 0.5 per
+    variable) and polynomial mutation (eta 20, probability 1/ndim per
+    variable), and keeps the best popsize of parents and offspring by front
+    then crowding distance.  These are pymoo's NSGA2 defaults, but the
+    random streams are this package's, so trajectories differ from the
+    reference's."""
+    max_obj = 3
+
+    def __init__(self, popsize, n_iter=2, seed=0, print_history=False, eta_c=15.0, prob_c=0.9, eta_m=20.0):
+        super().__init__(popsize, n_iter, seed, print_history)
+        self.eta_c, self.prob_c, self.eta_m = eta_c, prob_c, eta_m
+
+    def _start_algorithm(self):
+        self._pop_X = self._pop_F = None
+
+    def _tournament(self, rank, crowd, n):
+        a = self._rng.integers(0, len(rank), n)
+        b = self._rng.integers(0, len(rank), n)
+        better_a = (rank[a] < rank[b]) | ((rank[a] == rank[b]) & (crowd[a] > crowd[b]))
+        tie = (rank[a] == rank[b]) & (crowd[a] == crowd[b])
+        coin = self._rng.random(n) < 0.5
+        return np.where(better_a | (tie & coin), a, b)
+
+    def _sbx(self, p1, p2):
+        """Simulated binary crossover bounded to [0, 1] (Deb and Agrawal 1995;
+        Deb's bounded form)."""
+        c1, c2 = p1.copy(), p2.copy()
+        eta = self.eta_c
+        for i in range(len(p1)):
+            if self._rng.random() >= self.prob_c:
+                continue
+            for k in range(p1.shape[1]):
+                if self._rng.random() > 0.5 or abs(p1[i, k] - p2[i, k]) <= 1e-14:
+                    continue
+                y1, y2 = min(p1[i, k], p2[i, k]), max(p1[i, k], p2[i, k])
+                u = self._rng.random()
+                out = []
+                for beta in (1 + 2 * y1 / (y2 - y1), 1 + 2 * (1 - y2) / (y2 - y1)):
+                    alpha = 2 - beta ** -(eta + 1)
+                    bq = (u * alpha) ** (1 / (eta + 1)) if u <= 1 / alpha else (1 / (2 - u * alpha)) ** (1 / (eta + 1))
+                    out.append(bq)
+                v1 = min(max(0.5 * (y1 + y2 - out[0] * (y2 - y1)), 0.0), 1.0)
+                v2 = min(max(0.5 * (y1 + y2 + out[1] * (y2 - y1)), 0.0), 1.0)
+                if self._rng.random() < 0.5:
+                    v1, v2 = v2, v1
+                c1[i, k], c2[i, k] = v1, v2
+        return c1, c2
+
+    def _mutate(self, X):
+        """Polynomial mutation bounded to [0, 1] (Deb and Goyal 1996)."""
+        X = X.copy()
+        eta, pm = self.eta_m, 1.0 / X.shape[1]
+        for i in range(X.shape[0]):
+            for k in range(X.shape[1]):
+                if self._rng.random() >= pm:
+                    continue
+                y, u = X[i, k], self._rng.random()
+                if u < 0.5:
+                    val = 2 * u + (1 - 2 * u) * (1 - y) ** (eta + 1)
+                    dq = val ** (1 / (eta + 1)) - 1
+                else:
+                    val = 2 * (1 - u) + 2 * (u - 0.5) * y ** (eta + 1)
+                    dq = 1 - val ** (1 / (eta + 1))
+                X[i, k] = min(max(y + dq, 0.0), 1.0)
+        return X
+
+    def _ask(self):
+        if self._pop_X is None:
+            return self._rng.uniform(0, 1, (self.popsize, self.problem.ndim))
+        rank, crowd = _rank_and_crowding(self._pop_F)
+        n_pairs = (self.popsize + 1) // 2
+        p1 = self._pop_X[self._tournament(rank, crowd, n_pairs)]
+        p2 = self._pop_X[self._tournament(rank, crowd, n_pairs)]
+        c1, c2 = self._sbx(p1, p2)
+        return self._mutate(np.vstack([c1, c2])[:self.popsize])
+
+    def _tell_algorithm(self, X, F):
+        F = np.reshape(F, (len(X), -1))
+        if self._pop_X is None:
+            self._pop_X, self._pop_F = X.copy(), F.copy()
+            return
+        X_all, F_all = np.vstack([self._pop_X, X]), np.vstack([self._pop_F, F])
+        rank, crowd = _rank_and_crowding(F_all)
+        keep = np.lexsort((-crowd, rank))[:self.popsize]
+        self._pop_X, self._pop_F = X_all[keep], F_all[keep]
+
+    def _opt_X(self):
+        if self.problem.n_obj == 1:
+            return super()._opt_X()
+        return self._pop_X[_nondominated_ranks(self._pop_F) == 0]
+
+
 def batch_optimize(optimizers, problems, save=True):
-    """Run several CMA-ES optimisations together (optimize.py:1506-1715):
+    """Run several CMA-ES / NSGA-II optimisations together (optimize.py:1506-1715):
     each generation, the candidates of every optimisation still running are
     simulated in one call through a MultiSimGroup (SCs merged, duplicates
     shared), then each optimiser scores its own part and takes its next
@@ -488,8 +670,9 @@ def batch_optimize(optimizers, problems, save=True):
         problems = [problems] * len(optimizers)
     if len(optimizers) != len(problems):
         raise ValueError("Same number of optimizers and problems must be provided.")
-    if not all(isinstance(o, CMAESOptimizer) for o in optimizers):
-        raise TypeError("Batch optimization is only supported for population-based optimizers (CMAESOptimizer)")
+    if not all(isinstance(o, _PopulationOptimizer) for o in optimizers):
+        raise TypeError("Batch optimization is only supported for population-based optimizers "
+                        "(CMAESOptimizer, NSGA2Optimizer)")
     if len({type(p.sim_group) for p in problems}) != 1:
         raise ValueError("All optimizations must be done on the same type of model.")
     if save and any(p.sim_group.out_dir is None for p in problems):

@@ -21,7 +21,10 @@ FIC penalty enters the cost, and the grid's rows equal a direct SimGroup run.
 batch_optimize (one simulation call per generation for several optimisers)
 gives exactly the histories of the same optimisers run alone, on one problem
 and on two problems with different SCs, and raises the reference's argument
-errors.
+errors.  save() writes the reference's run-directory layout.  NSGA-II: its
+non-dominated sorting against the definition, crowding distance on a
+hand-worked front, convergence to the Pareto set of ZDT1, and the
+multi-objective BNMProblem path.
 """
 import os
 
@@ -287,3 +290,73 @@ def test_optimizer_save_layout(oracle_core, tmp_path):
         CMAESOptimizer(popsize=4).save()
     b = batch_optimize(cm, [p, p])
     assert os.path.isdir(tmp_path / "cmaes_run-1") and os.path.isdir(tmp_path / "cmaes_run-2") and len(b) == 2
+
+
+def test_nondominated_ranks_and_crowding():
+    """Fast non-dominated sorting against the definition (a row's front is one
+    more than the highest front among the rows dominating it, 0 if none),
+    on random sets with ties; crowding distance on a hand-worked front."""
+    from cubnm_amd.optimize import _crowding_distance, _nondominated_ranks
+    rng = np.random.default_rng(5)
+    for n, m in ((1, 2), (7, 2), (30, 2), (25, 3)):
+        F = rng.integers(0, 5, (n, m)).astype(float)          # many ties and duplicates
+        rank = _nondominated_ranks(F)
+        for i in range(n):
+            dominators = [j for j in range(n) if np.all(F[j] <= F[i]) and np.any(F[j] < F[i])]
+            assert rank[i] == (1 + max(rank[j] for j in dominators) if dominators else 0)
+    d = _crowding_distance(np.array([[0.0, 4.0], [1.0, 2.0], [2.0, 1.0], [4.0, 0.0]]))
+    assert np.isinf(d[0]) and np.isinf(d[3])
+    np.testing.assert_allclose(d[1:3], [2 / 4 + 3 / 4, 3 / 4 + 2 / 4])
+
+
+class _Zdt1:
+    """Two-objective test problem (Zitzler, Deb and Thiele 2000, ZDT1 with two
+    variables): f1 = x0, f2 = g (1 - sqrt(x0 / g)), g = 1 + 9 x1; the Pareto
+    front is x1 = 0."""
+    n_obj, ndim, multiobj = 2, 2, True
+    obj_names = ["f1", "f2"]
+    free_params = ["x0", "x1"]
+
+    def _get_Xt(self, X):
+        return np.asarray(X)
+
+    def evaluate(self, X):
+        g = 1 + 9 * X[:, 1]
+        return np.stack([X[:, 0], g * (1 - np.sqrt(X[:, 0] / g))], axis=1), None
+
+
+def test_nsga2_converges_on_zdt1():
+    from cubnm_amd.optimize import NSGA2Optimizer, _nondominated_ranks
+    o = NSGA2Optimizer(popsize=24, n_iter=40, seed=3)
+    o.setup_problem(_Zdt1())
+    assert o.optimize() is None                               # several objectives: no single optimum
+    h = o.history
+    assert len(h) == 24 * 41 and list(h.columns[1:]) == ["x0", "x1", "f1", "f2", "gen"]
+    assert ((h[["x0", "x1"]] >= 0) & (h[["x0", "x1"]] <= 1)).all().all()
+    assert h.loc[h["gen"] == 0, "x1"].mean() > 0.3
+    front = o._pop_X[_nondominated_ranks(o._pop_F) == 0]
+    assert len(front) >= 12 and front[:, 1].max() < 0.05      # the population reached the Pareto set
+    assert np.ptp(front[:, 0]) > 0.5                           # and spread along it
+    with pytest.raises(ValueError, match="at most 3"):
+        p4 = _Zdt1()
+        p4.n_obj = 4
+        o.setup_problem(p4)
+
+
+def test_nsga2_multiobjective_problem_and_batch(oracle_core, tmp_path):
+    """NSGA-II on a multi-objective BNMProblem: objective columns are the
+    problem's obj_names (-(+fc_corr) and -(-fcd_ks)); a batch of two seeds equals
+    the seeds run alone; save reruns the final non-dominated set."""
+    import copy
+    from cubnm_amd.optimize import NSGA2Optimizer, batch_optimize, _nondominated_ranks
+    p = _problem({"G": (0.5, 3.0), "w_p": (0.5, 1.5)}, multiobj=True, out_dir=str(tmp_path))
+    o1, o2 = NSGA2Optimizer(popsize=4, n_iter=1, seed=1), NSGA2Optimizer(popsize=4, n_iter=1, seed=2)
+    b = batch_optimize([o1, o2], p, save=False)
+    for o, k in ((o1, 0), (o2, 1)):
+        o.setup_problem(copy.deepcopy(p))
+        o.optimize()
+        assert _history_equal(b[k].history, o.history)
+    assert list(o1.history.columns[3:5]) == p.obj_names
+    d = o1.save()
+    with np.load(os.path.join(d, "opt_sim", "sim_data.npz")) as z:
+        assert z["sim_bold"].shape[0] == int((_nondominated_ranks(o1._pop_F) == 0).sum())
