@@ -7,6 +7,8 @@
   keeps every candidate inside the bounds and reports the best row.
 * batch_optimize of one CMA-ES over two problems with different SCs: one
   simulation call per generation, histories equal to independent runs.
+* NSGA-II on the two GOF objectives: its recorded objectives equal a direct
+  evaluation of the same candidates.
 The arithmetic of both against the oracle is in tests/test_optimize.py.
 """
 import numpy as np
@@ -78,3 +80,20 @@ def test_batch_optimize_variable_sc_matches_independent_runs():
         assert b[k].history.shape == alone.history.shape
         np.testing.assert_allclose(b[k].history.values.astype(float), alone.history.values.astype(float),
                                    rtol=0, atol=1e-6)
+
+
+def test_nsga2_multiobjective_runs():
+    """NSGA-II on the two GOF objectives: objective columns are the problem's
+    obj_names, candidates stay in bounds, the values equal a direct score of
+    the same candidates."""
+    from cubnm_amd.optimize import NSGA2Optimizer
+    p = _problem(multiobj=True)
+    o = NSGA2Optimizer(popsize=4, n_iter=1, seed=2)
+    o.setup_problem(p)
+    o.optimize()
+    h = o.history
+    assert len(h) == 8 and list(h.columns[3:5]) == p.obj_names
+    assert ((h["G"] >= 0.5) & (h["G"] <= 3.0)).all()
+    X = p._get_X(h.loc[h["gen"] == 1, ["G", "w_p"]].values)
+    F, _ = p.evaluate(X)
+    np.testing.assert_allclose(F, h.loc[h["gen"] == 1, p.obj_names].values, rtol=0, atol=1e-12)
