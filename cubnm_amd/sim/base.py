@@ -598,3 +598,20 @@ class MultiSimGroup:
             c.it += 1
             c._process_out(part)
             start = end
+
+
+def create_multi_sim_group(sim_group_cls):
+    """A MultiSimGroup class for one SimGroup class (sim/base.py:1440-1462):
+    its instances take a list of ``sim_group_cls`` instances and run them as
+    one call."""
+    if not (isinstance(sim_group_cls, type) and issubclass(sim_group_cls, SimGroup)):
+        raise TypeError("sim_group_cls must be a SimGroup subclass")
+
+    class _Multi(MultiSimGroup):
+        def __init__(self, sim_groups):
+            if not all(isinstance(g, sim_group_cls) for g in sim_groups):
+                raise TypeError(f"all simulation groups must be {sim_group_cls.__name__} instances")
+            super().__init__(sim_groups)
+
+    _Multi.__name__ = _Multi.__qualname__ = f"Multi{sim_group_cls.__name__}"
+    return _Multi

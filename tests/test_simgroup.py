@@ -198,6 +198,17 @@ def test_multi_sim_group_split(oracle_core):
         g.run()
         np.testing.assert_array_equal(g.sim_bold, bold)
         np.testing.assert_array_equal(g.sim_fc_trils, fc)
+    # the class factory of the reference's batch optimisation
+    from cubnm_amd.sim import JRSimGroup, create_multi_sim_group
+    MultiWC = create_multi_sim_group(WCSimGroup)
+    assert MultiWC.__name__ == "MultiWCSimGroup" and issubclass(MultiWC, MultiSimGroup)
+    m2 = MultiWC(groups[:2])
+    m2.run()
+    assert oracle_core[-1][14] == 3                               # N_SIMS of the merged call
+    with pytest.raises(TypeError):
+        MultiWC([JRSimGroup(sc=sc_a, **kw)])
+    with pytest.raises(TypeError):
+        create_multi_sim_group(int)
 
 
 def test_avail_gpus_counts_gpu_topology_nodes(tmp_path, monkeypatch):
