@@ -405,11 +405,17 @@ class _PopulationOptimizer(_Saving):
     one simulation call.  A subclass provides _start_algorithm, _ask and
     _tell_algorithm; history rows are the candidates' parameters, their
     objective values (column 'cost' for one objective, the problem's
-    obj_names otherwise), with one objective the score columns, and 'gen'."""
+    obj_names otherwise), with one objective the score columns, and 'gen'.
+    Keyword arguments of the reference's PymooOptimizer (optimize.py:1267-1330):
+    n_iter, seed, print_history, and save_history_sim (False: each
+    generation's simulation outputs are cleared once scored); a pymoo
+    ``termination`` object is not provided."""
 
-    def __init__(self, popsize, n_iter=2, seed=0, print_history=False):
+    def __init__(self, popsize, n_iter=2, seed=0, print_history=False, save_history_sim=False, termination=None):
+        if termination is not None:
+            raise NotImplementedError("termination: pymoo termination objects are not provided; use n_iter")
         self.popsize, self.n_iter, self.seed = popsize, n_iter, seed
-        self.print_history = print_history
+        self.print_history, self.save_history_sim = print_history, save_history_sim
 
     def setup_problem(self, problem):
         if problem.n_obj > self.max_obj:
@@ -443,6 +449,9 @@ class _PopulationOptimizer(_Saving):
         self._tell_algorithm(X, F)
         self._record(X, F, scores, self._gen)
         self._gen += 1
+        sg = getattr(self.problem, "sim_group", None)
+        if sg is not None and not self.save_history_sim:   # the generation's outputs are not kept
+            sg.clear()
 
     def _finish(self):
         self.history = pd.concat(self.history, axis=0).reset_index(drop=False)
@@ -469,11 +478,16 @@ class CMAESOptimizer(_PopulationOptimizer):
     popsize candidates per generation for n_iter generations after it.
     Uses this package's CMA-ES (fit.CMAES, Hansen's tutorial defaults,
     samples clipped to the cube) instead of pymoo's wrapper of `cma`, so the
-    trajectories are not the reference's."""
+    trajectories are not the reference's; hence ``use_bound_penalty`` and
+    ``algorithm_kws`` (options of `cma`) raise NotImplementedError."""
     max_obj = 1
 
-    def __init__(self, popsize, n_iter=2, seed=0, sigma=0.5, x0=None, print_history=False):
-        super().__init__(popsize, n_iter, seed, print_history)
+    def __init__(self, popsize, x0=None, sigma=0.5, use_bound_penalty=False, algorithm_kws=None, **kwargs):
+        if use_bound_penalty:
+            raise NotImplementedError("use_bound_penalty: candidates are clipped to the bounds instead")
+        if algorithm_kws:
+            raise NotImplementedError("algorithm_kws: options of the `cma` package are not provided")
+        super().__init__(popsize, **kwargs)
         self.sigma, self.x0 = sigma, x0
 
     def setup_problem(self, problem):
@@ -561,12 +575,15 @@ class NSGA2Optimizer(_PopulationOptimizer):
     variable), and keeps the best popsize of parents and offspring by front
     then crowding distance.  These are pymoo's NSGA2 defaults, but the
     random streams are this package's, so trajectories differ from the
-    reference's."""
+    reference's.  ``algorithm_kws`` may set eta_c, prob_c and eta_m."""
     max_obj = 3
 
-    def __init__(self, popsize, n_iter=2, seed=0, print_history=False, eta_c=15.0, prob_c=0.9, eta_m=20.0):
-        super().__init__(popsize, n_iter, seed, print_history)
-        self.eta_c, self.prob_c, self.eta_m = eta_c, prob_c, eta_m
+    def __init__(self, popsize, algorithm_kws=None, **kwargs):
+        kws = dict(algorithm_kws or {})
+        self.eta_c, self.prob_c, self.eta_m = kws.pop("eta_c", 15.0), kws.pop("prob_c", 0.9), kws.pop("eta_m", 20.0)
+        if kws:
+            raise NotImplementedError(f"algorithm_kws {sorted(kws)}: only eta_c, prob_c and eta_m are provided")
+        super().__init__(popsize, **kwargs)
 
     def _start_algorithm(self):
         self._pop_X = self._pop_F = None

@@ -360,3 +360,29 @@ def test_nsga2_multiobjective_problem_and_batch(oracle_core, tmp_path):
     d = o1.save()
     with np.load(os.path.join(d, "opt_sim", "sim_data.npz")) as z:
         assert z["sim_bold"].shape[0] == int((_nondominated_ranks(o1._pop_F) == 0).sum())
+
+
+def test_population_optimizer_arguments_and_history_sims(oracle_core):
+    """Constructor arguments in the reference's order and meaning
+    (optimize.py:1267-1330, 1386-1410, 1487-1504); without save_history_sim
+    each generation's simulation outputs are cleared once scored."""
+    from cubnm_amd.optimize import CMAESOptimizer, NSGA2Optimizer
+    o = CMAESOptimizer(4, None, 0.3, n_iter=1, seed=1)
+    assert (o.popsize, o.x0, o.sigma, o.n_iter, o.seed, o.save_history_sim) == (4, None, 0.3, 1, 1, False)
+    for bad in (dict(use_bound_penalty=True), dict(algorithm_kws={"CMA_elitist": True}), dict(termination=object())):
+        with pytest.raises(NotImplementedError):
+            CMAESOptimizer(4, **bad)
+    n = NSGA2Optimizer(6, {"eta_c": 5.0})
+    assert (n.eta_c, n.prob_c, n.eta_m) == (5.0, 0.9, 20.0)
+    with pytest.raises(NotImplementedError, match="mutation"):
+        NSGA2Optimizer(6, {"mutation": None})
+    p = _problem({"G": (0.5, 3.0)})
+    o.setup_problem(p)
+    o.optimize()
+    assert not hasattr(p.sim_group, "sim_bold")
+    q = _problem({"G": (0.5, 3.0)})
+    k = CMAESOptimizer(4, None, 0.3, n_iter=1, seed=1, save_history_sim=True)
+    k.setup_problem(q)
+    k.optimize()
+    assert q.sim_group.sim_bold.shape[0] == 4
+    np.testing.assert_array_equal(o.history.values, k.history.values)
