@@ -1,14 +1,13 @@
 """Loads the compiled core and initialises its session constants on import,
 as the reference's src/cubnm/_core.py:9-17 does for ``cubnm.core``.
 
-There is no fallback: if the extension is missing the import fails loudly.
+The package import has already compiled the core if it was missing or stale;
+there is no fallback: if the extension cannot be loaded the import fails loudly.
 """
-try:
-    from . import core
-except ImportError as e:  # pragma: no cover - exercised only on unbuilt trees
-    raise ImportError(
-        "cubnm_amd.core is not built; run `python -m cubnm_amd.build` (hipcc, gfx950)"
-    ) from e
+from . import build as _build
+
+_build.ensure_built()
+from . import core  # noqa: E402
 
 run_simulations = core.run_simulations
 set_const = core.set_const

@@ -32,9 +32,8 @@ def lib():
     global _LIB
     if _LIB is not None:
         return _LIB
-    path = os.path.join(_HERE, "libcubnm_amd.so")
-    if not os.path.exists(path):
-        raise ImportError("libcubnm_amd.so is not built; run `python -m cubnm_amd.build`")
+    from . import build as _build
+    path = _build.ensure_built()
     L = ctypes.CDLL(path)
     I, D, P, C = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_char_p
     CP = ctypes.POINTER(ctypes.c_char_p)
