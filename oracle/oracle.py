@@ -24,28 +24,50 @@ def build():
     subprocess.run(["make", "-C", _HERE, "-s", "liboracle.so"], check=True)
 
 
+def _bind(L):
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    D = ctypes.c_double
+    L.oracle_last_error.restype = ctypes.c_char_p
+    L.oracle_dims.argtypes = [ctypes.c_char_p, ctypes.c_char_p, I, I, I, I, D, D, P, P, P, P]
+    L.oracle_dims.restype = I
+    L.oracle_run.argtypes = ([ctypes.c_char_p, ctypes.c_char_p, I, I, I, P, P, P, P, P, P,
+                              I, I, I, I, I, I, D, D, P, I] + [P] * 10)
+    L.oracle_run.restype = I
+    L.oracle_fic_root.argtypes = [D]
+    L.oracle_fic_root.restype = D
+    L.oracle_corr.argtypes = [P, P, ctypes.c_long]
+    L.oracle_corr.restype = D
+    return L
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(path)
-        P = ctypes.c_void_p
-        I = ctypes.c_int
-        D = ctypes.c_double
-        L.oracle_last_error.restype = ctypes.c_char_p
-        L.oracle_dims.argtypes = [ctypes.c_char_p, ctypes.c_char_p, I, I, I, I, D, D, P, P, P, P]
-        L.oracle_dims.restype = I
-        L.oracle_run.argtypes = ([ctypes.c_char_p, ctypes.c_char_p, I, I, I, P, P, P, P, P, P,
-                                  I, I, I, I, I, I, D, D, P, I] + [P] * 10)
-        L.oracle_run.restype = I
-        L.oracle_fic_root.argtypes = [D]
-        L.oracle_fic_root.restype = D
-        L.oracle_corr.argtypes = [P, P, ctypes.c_long]
-        L.oracle_corr.restype = D
-        _LIB = L
+        _LIB = _bind(ctypes.CDLL(path))
     return _LIB
+
+
+def build_variant(source_text, out_dir, name="liboracle_variant.so"):
+    """Compile a modified copy of oracle.cpp (same flags as the Makefile) into
+    out_dir and return it bound like lib().  Tests use this to plant a mistake
+    in the oracle and show that a pin rejects it."""
+    import subprocess
+    src = os.path.join(out_dir, "oracle_variant.cpp")
+    with open(src, "w") as fh:
+        fh.write(source_text)
+    so = os.path.join(out_dir, name)
+    subprocess.run(["g++", "-std=c++11", "-O3", "-m64", "-fopenmp", "-fPIC", "-DNOISE_SEGMENT", "-shared",
+                    "-o", so, src], check=True)
+    return _bind(ctypes.CDLL(so))
+
+
+def source():
+    with open(os.path.join(_HERE, "oracle.cpp")) as fh:
+        return fh.read()
 
 
 def _ptr(a):
@@ -56,7 +78,7 @@ def config_string(config):
     return ";".join(f"{k}={v}" for k, v in config.items())
 
 
-def dims(model, config, nodes, time_steps, BOLD_TR, states_sampling, dt, bw_dt):
+def dims(model, config, nodes, time_steps, BOLD_TR, states_sampling, dt, bw_dt, lib=lib):
     out = np.zeros(16, np.int32)
     ns = np.zeros(1, np.int64)
     rc = lib().oracle_dims(model.encode(), config_string(config).encode(), nodes, time_steps, BOLD_TR,
@@ -79,14 +101,17 @@ def dims(model, config, nodes, time_steps, BOLD_TR, states_sampling, dt, bw_dt):
 
 def run(model, SC, SC_indices, SC_dist, global_params, regional_params, v_list, config,
         states_ts, noise_out, do_delay, N_SIMS, nodes, time_steps, BOLD_TR, states_sampling,
-        sim_seed, dt, bw_dt, bw_consts=FRISTON2003, n_threads=0):
+        sim_seed, dt, bw_dt, bw_consts=FRISTON2003, n_threads=0, lib=lib):
     """Run simulations on the CPU oracle.
 
     ``global_params`` and ``regional_params`` are float64 C-contiguous arrays
     modified in place (analytical/numerical FIC writes wIE), as the reference
     does when the model modifies parameters (core.cpp:437-450).
     """
-    d = dims(model, config, nodes, time_steps, BOLD_TR, states_sampling, dt, bw_dt)
+    if isinstance(lib, ctypes.CDLL):
+        _variant = lib
+        lib = lambda: _variant  # noqa: E731
+    d = dims(model, config, nodes, time_steps, BOLD_TR, states_sampling, dt, bw_dt, lib=lib)
     SC = np.ascontiguousarray(SC, np.float64).reshape(-1, nodes * nodes)
     SC_indices = np.ascontiguousarray(SC_indices, np.int32)
     SC_dist = np.ascontiguousarray(SC_dist, np.float64).reshape(-1)

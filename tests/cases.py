@@ -116,14 +116,14 @@ def sim_args(model, sc, gp, rp, config, n_sims, duration_s=60, TR_s=1.0, dt="0.1
             sim_seed, float(dt), float(bw_dt))
 
 
-def run_oracle(args, bw_consts=FRISTON2003, n_threads=0):
+def run_oracle(args, bw_consts=FRISTON2003, n_threads=0, lib=None):
     """Run the same argument tuple through the CPU oracle (test infrastructure)."""
     from oracle import oracle as O
     (model, sc2, sci, dist, gp, rp, vv, config, ext_out, states_ts, noise_out, do_delay, _fr, _cpu,
      n_sims, nodes, time_steps, TR, ss, seed, dt, bw_dt) = args
     return O.run(model, sc2, sci, dist.reshape(nodes, nodes), gp, rp, vv, config, states_ts, noise_out,
                  do_delay, n_sims, nodes, time_steps, TR, ss, seed, dt, bw_dt, bw_consts=bw_consts,
-                 n_threads=n_threads)
+                 n_threads=n_threads, lib=O.lib if lib is None else lib)
 
 
 def copy_args(args):
