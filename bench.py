@@ -62,10 +62,33 @@ def ops_per_sim_node_second(nodes, dt_ms, bw_dt_ms, TR_s):
     return (1000.0 / dt_ms) * (2 * nodes + 42) + (1000.0 / bw_dt_ms) * 24 + 10.0 / TR_s
 
 
-def cpu_baseline(n_cores, nodes, duration_s, G, W, J, sc, cfg, per_core=3):
+def host_cpu():
+    """Threads the host lets this process run, and the CPU model.  The thread
+    count is the affinity mask, capped by the cgroup CPU quota when one is set
+    (a quota below the mask means more threads would only be throttled)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            n = max(1, min(n, int(-(-int(q) // int(period)))))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, model
+
+
+def cpu_baseline(n_cores, nodes, duration_s, G, W, J, sc, cfg, per_core=2, repeats=3):
     """The oracle (FP64 C++ restatement of the reference CPU engine, same
-    flags) on a bounded sample of the grid: per_core full-length simulations
-    per thread (about 10-15 s of CPU work on 16 cores), run phase only."""
+    flags, OpenMP over simulations) on a bounded sample of the grid:
+    per_core full-length simulations per thread, run phase only (noise
+    generation excluded), median of ``repeats`` runs."""
     import cases
     from oracle import oracle as O
     O.lib()
@@ -74,15 +97,18 @@ def cpu_baseline(n_cores, nodes, duration_s, G, W, J, sc, cfg, per_core=3):
     gp, rp = cases.params("rWW", n, nodes, G=G[pick], w_p=W[pick][:, None], J_N=J[pick][:, None],
                           do_fic=False, wIE=1.0)
     args = cases.sim_args("rWW", sc, gp, rp, cfg, n, duration_s=duration_s)
-    out = cases.run_oracle(args, n_threads=n_cores)
-    rt = out["run_time"]
+    rts = [cases.run_oracle(args, n_threads=n_cores)["run_time"] for _ in range(repeats)]
+    rt = float(np.median(rts))
+    _, model = host_cpu()
     return {
-        "value": n * nodes * duration_s / rt,
+        "value": round(n * nodes * duration_s / rt, 1),
         "unit": "sim-node-s/s",
         "cores": n_cores,
         "kind": "port",
+        "cpu_model": model,
         "sample": f"{n} of the grid's simulations (evenly spaced), full {duration_s:g} s each, "
-                  f"{per_core} per OpenMP thread; run phase only ({rt:.2f} s), noise generation excluded",
+                  f"{per_core} per OpenMP thread on {n_cores} threads; run phase only, noise generation "
+                  f"excluded; median of {repeats} runs ({', '.join(f'{r:.2f}' for r in rts)} s)",
     }
 
 
@@ -105,7 +131,8 @@ def main():
     ap.add_argument("--sims", type=int, default=32768)
     ap.add_argument("--nodes", type=int, default=100)
     ap.add_argument("--duration", type=float, default=60.0)
-    ap.add_argument("--cpu-cores", type=int, default=0, help="threads for the CPU baseline (0: all, max 16)")
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="threads for the CPU baseline (0: every core the process may use)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tpw", type=int, default=0, help="batched integrator tiles per wave (0: auto)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -216,7 +243,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        n_cores = args.cpu_cores or min(16, len(os.sched_getaffinity(0)))
+        n_cores = args.cpu_cores or host_cpu()[0]
         cpu = cpu_baseline(n_cores, N, dur, G, W, J, sc, cfg)
         log(f"cpu baseline: {cpu}")
 
