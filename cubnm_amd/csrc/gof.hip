@@ -8,13 +8,19 @@
 //   F_emp(x)| over the pooled sample points, with right-continuous ECDFs
 //   (reference: scipy.stats.ks_2samp(...).statistic, and utils.py:306-414
 //   fcd_ks_device with its right-searchsorted cdf_2d_2d_right kernel).
-//   Each workgroup sorts one simulation's FCD values in LDS (bitonic), then
-//   every pooled point evaluates both ECDFs by binary search; the empirical
-//   values arrive sorted from the host.  NaN anywhere in a row gives NaN.
+//   Any number of values per simulation: each simulation's FCD values are
+//   cut into chunks of at most 8,192, each chunk bitonic-sorted in LDS and
+//   written to a device workspace; then every pooled point (simulation and
+//   empirical values) counts the simulation values <= it by a binary search
+//   in each sorted chunk and the empirical values by one in the sorted
+//   empirical array, and the largest |c_sim m - c_emp n| over the points is
+//   reduced with an integer atomic max.  The empirical values arrive sorted.
+//   NaN anywhere in a row gives NaN.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -83,47 +89,76 @@ __global__ void __launch_bounds__(GOF_THREADS) fc_corr_kernel(const double* sim,
     }
 }
 
-__global__ void __launch_bounds__(GOF_THREADS) fcd_ks_kernel(const double* sim, int n_sims, int n, int n_pow2,
-                                                             const double* emp_sorted, int m, double* out) {
-    extern __shared__ double v[];  // [n_pow2]
-    __shared__ double red[GOF_THREADS / 64];
-    for (int s = blockIdx.x; s < n_sims; s += gridDim.x) {
-        const double* x = sim + (long long)s * n;
-        double nan_seen = 0.0;
-        for (int i = threadIdx.x; i < n_pow2; i += blockDim.x) {
-            double val = i < n ? x[i] : INFINITY;
-            if (isnan(val)) { nan_seen = 1.0; val = INFINITY; }
-            v[i] = val;
-        }
-        nan_seen = wg_max(nan_seen, red);
-        // bitonic sort, ascending
-        for (int k = 2; k <= n_pow2; k <<= 1) {
-            for (int jb = k >> 1; jb > 0; jb >>= 1) {
-                __syncthreads();
-                for (int i = threadIdx.x; i < n_pow2; i += blockDim.x) {
-                    const int p = i ^ jb;
-                    if (p > i) {
-                        const double a = v[i], b = v[p];
-                        const bool up = (i & k) == 0;
-                        if ((a > b) == up) { v[i] = b; v[p] = a; }
-                    }
+constexpr int KS_CHUNK = 8192;  // values per LDS sort (64 KB)
+
+// Sort chunk blockIdx.x of simulation blockIdx.y (of the batch starting at
+// sim0) into ws[s][chunk]; NaN values become +inf and mark the simulation.
+__global__ void __launch_bounds__(GOF_THREADS) ks_sort_chunks_kernel(const double* sim, int sim0, int n, int chunk_pow2,
+                                                                     double* ws, int* nan_flag) {
+    extern __shared__ double v[];  // [chunk_pow2]
+    const int s = blockIdx.y, c0 = blockIdx.x * chunk_pow2;
+    const int len = min(chunk_pow2, n - c0);
+    const double* x = sim + (long long)(sim0 + s) * n + c0;
+    bool nan_seen = false;
+    for (int i = threadIdx.x; i < chunk_pow2; i += blockDim.x) {
+        double val = i < len ? x[i] : INFINITY;
+        if (isnan(val)) { nan_seen = true; val = INFINITY; }
+        v[i] = val;
+    }
+    if (nan_seen) atomicOr(nan_flag + s, 1);
+    for (int k = 2; k <= chunk_pow2; k <<= 1) {
+        for (int jb = k >> 1; jb > 0; jb >>= 1) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < chunk_pow2; i += blockDim.x) {
+                const int p = i ^ jb;
+                if (p > i) {
+                    const double a = v[i], b = v[p];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) { v[i] = b; v[p] = a; }
                 }
             }
         }
-        __syncthreads();
-        // |F_sim - F_emp| = |c_s m - c_e n| / (n m): the integer numerator is
-        // exact, so equal ECDFs give exactly 0 and the one division rounds once
-        long long dmax = 0;
-        for (int i = threadIdx.x; i < n + m; i += blockDim.x) {
-            const double z = i < n ? v[i] : emp_sorted[i - n];
-            const long long cs = upper_bound(v, n, z), ce = upper_bound(emp_sorted, m, z);
-            const long long dd = cs * m - ce * n;
-            dmax = max(dmax, dd < 0 ? -dd : dd);
-        }
-        const double d = wg_max((double)dmax, red);   // integers below 2^53: exact in double
-        if (threadIdx.x == 0) out[s] = nan_seen > 0.0 ? NAN : d / ((double)n * (double)m);
-        __syncthreads();
     }
+    __syncthreads();
+    double* out = ws + (long long)s * n + c0;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) out[i] = v[i];
+}
+
+// Pooled points [blockIdx.x * per_block, ...) of simulation blockIdx.y:
+// dmax[s] = max |c_sim(z) m - c_emp(z) n| (exact integers).
+__global__ void __launch_bounds__(GOF_THREADS) ks_eval_kernel(const double* ws, int n, int chunk_pow2,
+                                                              const double* emp_sorted, int m, int per_block,
+                                                              unsigned long long* dmax) {
+    __shared__ double red[GOF_THREADS / 64];
+    const int s = blockIdx.y;
+    const double* w = ws + (long long)s * n;
+    const int n_chunks = (n + chunk_pow2 - 1) / chunk_pow2;
+    const int p0 = blockIdx.x * per_block, p1 = min(n + m, p0 + per_block);
+    long long best = 0;
+    for (int i = p0 + threadIdx.x; i < p1; i += blockDim.x) {
+        const double z = i < n ? w[i] : emp_sorted[i - n];
+        long long cs = 0;
+        for (int c = 0; c < n_chunks; c++) {
+            const int c0 = c * chunk_pow2;
+            cs += upper_bound(w + c0, min(chunk_pow2, n - c0), z);
+        }
+        const long long ce = upper_bound(emp_sorted, m, z);
+        const long long dd = cs * m - ce * n;
+        best = max(best, dd < 0 ? -dd : dd);
+    }
+    const double b = wg_max((double)best, red);  // integers below 2^53: exact in double
+    if (threadIdx.x == 0) atomicMax(dmax + s, (unsigned long long)b);
+}
+
+__global__ void ks_finish_kernel(int sim0, int n_batch, int n, int m, const unsigned long long* dmax,
+                                 const int* nan_flag, double* out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_batch) out[sim0 + s] = nan_flag[s] ? NAN : (double)dmax[s] / ((double)n * (double)m);
+}
+
+__global__ void ks_clear_kernel(int n_batch, unsigned long long* dmax, int* nan_flag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_batch) { dmax[s] = 0; nan_flag[s] = 0; }
 }
 
 }  // namespace cubnm
@@ -142,25 +177,70 @@ int guarded(F f) {
 }  // namespace
 
 namespace {
-template <class F>
-int with_device_copies(const double* sim, size_t sim_n, const double* emp, size_t emp_n, double* out, size_t out_n, F f) {
-    double *d_sim = nullptr, *d_emp = nullptr, *d_out = nullptr;
-    int rc = -1;
-    if (hipMalloc(&d_sim, sizeof(double) * sim_n) == hipSuccess && hipMalloc(&d_emp, sizeof(double) * emp_n) == hipSuccess &&
-        hipMalloc(&d_out, sizeof(double) * out_n) == hipSuccess &&
-        hipMemcpy(d_sim, sim, sizeof(double) * sim_n, hipMemcpyHostToDevice) == hipSuccess &&
-        hipMemcpy(d_emp, emp, sizeof(double) * emp_n, hipMemcpyHostToDevice) == hipSuccess) {
-        rc = f(d_sim, d_emp, d_out);
-        if (rc == 0 && hipMemcpy(out, d_out, sizeof(double) * out_n, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
-    } else {
-        g_gof_err = "device allocation or copy failed";
-        cubnm::set_last_error(g_gof_err);
+
+// Device buffers the GOF entry points reuse across calls, grown when a call
+// needs more and held until the process ends (the runtime may be gone by the
+// time static destructors run, so they are not freed then).  Calls are
+// serialised by a mutex; they run on the null stream and wait for it (not
+// for the whole device).
+struct GofBuffers {
+    std::mutex mu;
+    void* p[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t bytes[5] = {0, 0, 0, 0, 0};
+    hipError_t ensure(int i, size_t b) {
+        if (bytes[i] >= b) return hipSuccess;
+        if (p[i]) { (void)hipFree(p[i]); p[i] = nullptr; bytes[i] = 0; }
+        hipError_t e = hipMalloc(&p[i], b);
+        if (e == hipSuccess) bytes[i] = b;
+        return e;
     }
-    if (d_sim) hipFree(d_sim);
-    if (d_emp) hipFree(d_emp);
-    if (d_out) hipFree(d_out);
-    return rc;
+};
+GofBuffers& bufs() {
+    static GofBuffers b;
+    return b;
 }
+enum { BUF_SIM = 0, BUF_EMP = 1, BUF_OUT = 2, BUF_KS_WS = 3, BUF_KS_RED = 4 };
+
+hipError_t fc_corr_launch(const double* d_sim, int n_sims, int n_pairs, const double* d_emp, double* d_out) {
+    hipLaunchKernelGGL(cubnm::fc_corr_kernel, dim3(std::min(n_sims, 4096)), dim3(cubnm::GOF_THREADS), 0, 0,
+                       d_sim, n_sims, n_pairs, d_emp, d_out);
+    return hipGetLastError();
+}
+
+// KS over all simulations, in batches whose sorted copies fit the workspace
+// (at most 256 MB or one simulation).
+hipError_t fcd_ks_launch(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m, double* d_out) {
+    using namespace cubnm;
+    GofBuffers& B = bufs();
+    int chunk = 1;
+    while (chunk < std::min(n, KS_CHUNK)) chunk <<= 1;
+    const size_t per_sim = sizeof(double) * (size_t)n;
+    // grid.y carries the batch: at most 65,535 simulations per launch
+    const int batch = (int)std::max<size_t>(
+        1, std::min<size_t>({(size_t)n_sims, ((size_t)256 << 20) / per_sim, (size_t)65535}));
+    hipError_t e;
+    if ((e = B.ensure(BUF_KS_WS, per_sim * batch)) != hipSuccess) return e;
+    if ((e = B.ensure(BUF_KS_RED, (sizeof(unsigned long long) + sizeof(int)) * batch)) != hipSuccess) return e;
+    double* ws = static_cast<double*>(B.p[BUF_KS_WS]);
+    unsigned long long* dmax = static_cast<unsigned long long*>(B.p[BUF_KS_RED]);
+    int* nan_flag = reinterpret_cast<int*>(dmax + batch);
+    const int n_chunks = (n + chunk - 1) / chunk;
+    const int per_block = GOF_THREADS * 16;
+    const int eval_blocks = (n + m + per_block - 1) / per_block;
+    for (int s0 = 0; s0 < n_sims; s0 += batch) {
+        const int nb = std::min(batch, n_sims - s0);
+        hipLaunchKernelGGL(ks_clear_kernel, dim3((nb + 255) / 256), dim3(256), 0, 0, nb, dmax, nan_flag);
+        hipLaunchKernelGGL(ks_sort_chunks_kernel, dim3(n_chunks, nb), dim3(GOF_THREADS), sizeof(double) * chunk, 0,
+                           d_sim, s0, n, chunk, ws, nan_flag);
+        hipLaunchKernelGGL(ks_eval_kernel, dim3(eval_blocks, nb), dim3(GOF_THREADS), 0, 0, ws, n, chunk,
+                           d_emp_sorted, m, per_block, dmax);
+        hipLaunchKernelGGL(ks_finish_kernel, dim3((nb + 255) / 256), dim3(256), 0, 0, s0, nb, n, m, dmax, nan_flag,
+                           d_out);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -173,33 +253,38 @@ const char* cubnm_gof_last_error(void) { return g_gof_err.c_str(); }
 int cubnm_gof_fc_corr_device(const double* d_sim, int n_sims, int n_pairs, const double* d_emp, double* d_out) {
     if (n_sims <= 0 || n_pairs <= 1) { g_gof_err = "need n_sims > 0 and n_pairs > 1"; return -1; }
     return guarded([&]() -> hipError_t {
-        hipLaunchKernelGGL(cubnm::fc_corr_kernel, dim3(std::min(n_sims, 4096)), dim3(cubnm::GOF_THREADS), 0, 0,
-                           d_sim, n_sims, n_pairs, d_emp, d_out);
-        hipError_t e = hipGetLastError();
-        return e == hipSuccess ? hipDeviceSynchronize() : e;
+        hipError_t e = fc_corr_launch(d_sim, n_sims, n_pairs, d_emp, d_out);
+        return e == hipSuccess ? hipStreamSynchronize(0) : e;
     });
 }
 
 int cubnm_gof_fcd_ks_device(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m, double* d_out) {
     if (n_sims <= 0 || n <= 0 || m <= 0) { g_gof_err = "need n_sims, n and m > 0"; return -1; }
-    int n_pow2 = 1;
-    while (n_pow2 < n) n_pow2 <<= 1;
-    if ((size_t)n_pow2 * sizeof(double) > 128 * 1024) { g_gof_err = "too many FCD values per simulation for the LDS sort"; return -1; }
+    std::lock_guard<std::mutex> lock(bufs().mu);
     return guarded([&]() -> hipError_t {
-        hipLaunchKernelGGL(cubnm::fcd_ks_kernel, dim3(std::min(n_sims, 4096)), dim3(cubnm::GOF_THREADS),
-                           sizeof(double) * n_pow2, 0, d_sim, n_sims, n, n_pow2, d_emp_sorted, m, d_out);
-        hipError_t e = hipGetLastError();
-        return e == hipSuccess ? hipDeviceSynchronize() : e;
+        hipError_t e = fcd_ks_launch(d_sim, n_sims, n, d_emp_sorted, m, d_out);
+        return e == hipSuccess ? hipStreamSynchronize(0) : e;
     });
 }
 
-
 int cubnm_gof_fc_corr(const double* sim_fc, int n_sims, int n_pairs, const double* emp_fc, double* out) {
     if (n_sims <= 0 || n_pairs <= 1) { g_gof_err = "need n_sims > 0 and n_pairs > 1"; return -1; }
-    return with_device_copies(sim_fc, (size_t)n_sims * n_pairs, emp_fc, n_pairs, out, n_sims,
-                              [&](double* ds, double* de, double* dout) {
-                                  return cubnm_gof_fc_corr_device(ds, n_sims, n_pairs, de, dout);
-                              });
+    GofBuffers& B = bufs();
+    std::lock_guard<std::mutex> lock(B.mu);
+    return guarded([&]() -> hipError_t {
+        hipError_t e;
+        const size_t sb = sizeof(double) * (size_t)n_sims * n_pairs;
+        if ((e = B.ensure(BUF_SIM, sb)) != hipSuccess) return e;
+        if ((e = B.ensure(BUF_EMP, sizeof(double) * n_pairs)) != hipSuccess) return e;
+        if ((e = B.ensure(BUF_OUT, sizeof(double) * n_sims)) != hipSuccess) return e;
+        double *ds = static_cast<double*>(B.p[BUF_SIM]), *de = static_cast<double*>(B.p[BUF_EMP]);
+        double* dout = static_cast<double*>(B.p[BUF_OUT]);
+        if ((e = hipMemcpyAsync(ds, sim_fc, sb, hipMemcpyHostToDevice, 0)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(de, emp_fc, sizeof(double) * n_pairs, hipMemcpyHostToDevice, 0)) != hipSuccess) return e;
+        if ((e = fc_corr_launch(ds, n_sims, n_pairs, de, dout)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(out, dout, sizeof(double) * n_sims, hipMemcpyDeviceToHost, 0)) != hipSuccess) return e;
+        return hipStreamSynchronize(0);
+    });
 }
 
 int cubnm_gof_fcd_ks(const double* sim_fcd, int n_sims, int n, const double* emp_fcd, int m, double* out) {
@@ -207,10 +292,22 @@ int cubnm_gof_fcd_ks(const double* sim_fcd, int n_sims, int n, const double* emp
     std::vector<double> emp(emp_fcd, emp_fcd + m);
     for (double x : emp) if (std::isnan(x)) { for (int i = 0; i < n_sims; i++) out[i] = NAN; return 0; }
     std::sort(emp.begin(), emp.end());
-    return with_device_copies(sim_fcd, (size_t)n_sims * n, emp.data(), m, out, n_sims,
-                              [&](double* ds, double* de, double* dout) {
-                                  return cubnm_gof_fcd_ks_device(ds, n_sims, n, de, m, dout);
-                              });
+    GofBuffers& B = bufs();
+    std::lock_guard<std::mutex> lock(B.mu);
+    return guarded([&]() -> hipError_t {
+        hipError_t e;
+        const size_t sb = sizeof(double) * (size_t)n_sims * n;
+        if ((e = B.ensure(BUF_SIM, sb)) != hipSuccess) return e;
+        if ((e = B.ensure(BUF_EMP, sizeof(double) * m)) != hipSuccess) return e;
+        if ((e = B.ensure(BUF_OUT, sizeof(double) * n_sims)) != hipSuccess) return e;
+        double *ds = static_cast<double*>(B.p[BUF_SIM]), *de = static_cast<double*>(B.p[BUF_EMP]);
+        double* dout = static_cast<double*>(B.p[BUF_OUT]);
+        if ((e = hipMemcpyAsync(ds, sim_fcd, sb, hipMemcpyHostToDevice, 0)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(de, emp.data(), sizeof(double) * m, hipMemcpyHostToDevice, 0)) != hipSuccess) return e;
+        if ((e = fcd_ks_launch(ds, n_sims, n, de, m, dout)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(out, dout, sizeof(double) * n_sims, hipMemcpyDeviceToHost, 0)) != hipSuccess) return e;
+        return hipStreamSynchronize(0);
+    });
 }
 
 }  // extern "C"

@@ -428,8 +428,11 @@ int general_ksplit(int nodes, int want) {
 // 12.7 us, 256 simulations sharing one SC 21.4 -> 13.1 us, N=200 one
 // simulation 6.6 -> 4.3 us; but 256 simulations with one SC each (HBM-bound)
 // 43-45 -> 48 us, N=200 x 512 21.5 -> 23 us.
+int g_general_pair = 0;  // 0 = auto (latency-bound launches), 1 = whenever eligible, -1 = never
+
 bool general_pair_ok(const IntegrateArgs& a, int threads, bool latency_bound) {
-    return latency_bound && !a.do_delay && !a.sc_lds && a.nodes % 2 == 0 && a.nodes >= 96 &&
+    if (g_general_pair < 0) return false;
+    return (latency_bound || g_general_pair > 0) && !a.do_delay && !a.sc_lds && a.nodes % 2 == 0 && a.nodes >= 96 &&
            reinterpret_cast<uintptr_t>(a.SC) % 16 == 0 && 2 * general_npc(a.nodes) <= threads;
 }
 

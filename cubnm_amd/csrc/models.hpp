@@ -264,15 +264,15 @@ CUBNM_HD void bw_step(const BWConsts& b, double s, double& x, double& f, double&
 // This matches pow to a few ulp (the BW system is contracting, so the
 // difference stays at rounding level) and costs a third of the device pow.
 // Special values follow pow: log(0) = -inf -> 0, log(x<0) = NaN -> NaN.
-// nu^(1/alpha) for the friston2003 / heinzle alpha = 0.32 (1/alpha = 3.125
-// exactly in FP64): nu^3 * nu^(1/8) with three square roots, a few ulp from
-// pow; any other alpha takes exp(ialpha * log(nu)).
+// nu^(1/alpha) with the reference's alpha = 0.32 (bw.cpp:4-22; no entry point
+// changes it, and 1/alpha = 3.125 exactly in FP64): nu^3 * nu^(1/8) with
+// three square roots, a few ulp from pow.  There is no general-alpha path:
+// its log/exp coefficients, kept in registers across the integrators' loops,
+// cost more registers than the rest of the BW step.
 __device__ inline double bw_nu_pow(const BWConsts& b, double nu) {
-    if (b.ialpha == 3.125) {
-        const double r8 = sqrt(sqrt(sqrt(nu)));
-        return (nu * nu) * nu * r8;
-    }
-    return exp(b.ialpha * log(nu));
+    (void)b;
+    const double r8 = sqrt(sqrt(sqrt(nu)));
+    return (nu * nu) * nu * r8;
 }
 
 // The BW step of bw_step_explog with fewer instructions: powers of nu as

@@ -40,6 +40,7 @@ void batched_plan(int nodes, int want_tpw, int* tpw, int* nt_pad);
 size_t batched_afrag_doubles(int nodes, int nt_pad, int n_scs);
 hipError_t build_afrag(const double* d_SC, int n_scs, int nodes, int nt_pad, double* d_afrag, hipStream_t s);
 extern int g_batched_tpw;
+extern int g_general_pair;
 
 namespace {
 
@@ -866,6 +867,11 @@ int cubnm_set_tuning(const char* key, int value) {
     if (std::strcmp(key, "general_ksplit") == 0) {
         if (value < 0 || value > 8) { g_err = "general_ksplit must be 0 (auto) or 1..8"; return -1; }
         g_general_ksplit = value;
+        return 0;
+    }
+    if (std::strcmp(key, "general_pair") == 0) {
+        if (value < -1 || value > 1) { g_err = "general_pair must be -1 (never), 0 (auto) or 1 (whenever eligible)"; return -1; }
+        g_general_pair = value;
         return 0;
     }
     if (std::strcmp(key, "fc_mode") == 0) {

@@ -152,6 +152,11 @@ int cubnm_last_integrator(void);
  *                   coupling sum over source nodes (0 = auto, 1..8; capped
  *                   at 1024 lanes per workgroup); the partial sums are added
  *                   in partition order;
+ *   "general_pair" = per-simulation integrator sums two adjacent SC columns
+ *                   per lane with 16-byte loads: 0 = auto (launches bound by
+ *                   latency: SCs shared or few simulations), 1 = whenever
+ *                   eligible (even N >= 96, SC in global memory, no delay),
+ *                   -1 = never;
  *   "fc_mode"     = FC/FCD path: 0 = auto (per-simulation LDS kernel when a
  *                   simulation's BOLD fits in LDS, else FP64 Gram products),
  *                   1 = FP64 Gram products on MFMA, 2 = split-BF16 Gram

@@ -1,0 +1,105 @@
+"""Benchmark-configuration corners and Balloon-Windkessel constants on the GPU.
+
+* The 32x32x32 rWW grid's parameter corners (G in {0.001, 10} x J_N in
+  {0.001, 0.5} x w_p in {0, 2}; N = 100, 60 s, dt 0.1 ms, TR 1 s, FCD window
+  10 step 2, FIC off) through the batched integrator and the per-simulation
+  integrator (automatic and four k partitions) against the oracle: BOLD
+  norm-wise relative error <= 1e-9 per simulation, FC and FCD absolute <=
+  1e-9, with NaN required in the same places.  These corners are where the
+  excitatory population saturates (S_E clipped at 1) or nearly decouples.
+* set_const: Kuramoto with omega = G = sigma = 0 holds every phase at its
+  initial value, so each node's BW integrator sees a constant input s; after
+  200 s its BOLD equals the closed form V0 [k1 (1-q) + k2 (1 - q/nu) +
+  k3 (1 - nu)] with f = 1 + s/gamma, nu = f^alpha, q = nu (1 - (1-rho)^(1/f))
+  / rho.  The core must give that value under the heinzle2016-3T constants
+  and a changed V_0 set through set_const (core.cpp:157-192), and the
+  default constants again after set_const restores them.
+"""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def _corner_args():
+    N = 100
+    sc = cases.synthetic_sc(N, seed=0)
+    G, J, W = np.meshgrid([0.001, 10.0], [0.001, 0.5], [0.0, 2.0], indexing="ij")
+    G, J, W = G.ravel(), J.ravel(), W.ravel()
+    gp, rp = cases.params("rWW", 8, N, G=G, w_p=W[:, None], J_N=J[:, None], do_fic=False, wIE=1.0)
+    cfg = cases.model_config("rWW", do_fic=False)
+    return cases.sim_args("rWW", sc, gp, rp, cfg, 8, duration_s=60)
+
+
+@pytest.fixture(scope="module")
+def corner_reference():
+    return cases.run_oracle(cases.copy_args(_corner_args()))
+
+
+@pytest.mark.parametrize("integrator", ["batched", "general", "general_ks4"])
+def test_c2_grid_corners(corner_reference, integrator):
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    _core.force_integrator(1 if integrator == "batched" else 0)
+    _core.core.set_tuning("general_ksplit", 4 if integrator == "general_ks4" else 0)
+    try:
+        out = _core.run_simulations(*cases.copy_args(_corner_args()))
+        assert _core.core.last_timings()["integrator"] == (1 if integrator == "batched" else 0)
+    finally:
+        _core.force_integrator(-1)
+        _core.core.set_tuning("general_ksplit", 0)
+    ref = corner_reference
+    for k in ["sim_bold", "sim_fc_trils", "sim_fcd_trils"]:
+        np.testing.assert_array_equal(np.isnan(out[k]), np.isnan(ref[k]), err_msg=k)
+    rel = cases.bold_rel_err(out["sim_bold"], ref["sim_bold"])
+    assert rel.max() <= TOL, rel
+    assert cases.max_abs_err(out["sim_fc_trils"], ref["sim_fc_trils"]) <= TOL
+    assert cases.max_abs_err(out["sim_fcd_trils"], ref["sim_fcd_trils"]) <= TOL
+    s, r = out["_sim_states"], ref["_sim_states"]
+    scale = np.maximum(np.nanmax(np.abs(r), axis=(1, 2), keepdims=True), 1e-300)
+    assert cases.max_abs_err(s / scale, r / scale) <= TOL
+
+
+RHO, ALPHA, GAMMA = 0.34, 0.32, 1 / 0.41
+
+
+def _closed_form(s, k1, k2, k3, V0):
+    f = 1 + s / GAMMA
+    nu = f ** ALPHA
+    q = nu * (1 - (1 - RHO) ** (1 / f)) / RHO
+    return V0 * (k1 * (1 - q) + k2 * (1 - q / nu) + k3 * (1 - nu))
+
+
+@pytest.mark.parametrize("integrator", ["batched", "general"])
+def test_set_const_changes_bold_constants(integrator):
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    theta = np.array([0.05, 0.3, 0.7, 1.5, 3.0, -0.2])
+    N = theta.size
+
+    def run():
+        gp, rp = cases.params("Kuramoto", 1, N, G=[0.0], init_theta=theta[None, :], omega=0.0, sigma=0.0)
+        cfg = cases.model_config("Kuramoto", window_size=2, window_step=1, bold_remove_s=2)
+        args = cases.sim_args("Kuramoto", cases.synthetic_sc(N), gp, rp, cfg, 1, duration_s=200, dt="1.0")
+        return _core.run_simulations(*args)["sim_bold"].reshape(-1, N)[-1]
+
+    friston = cases.FRISTON2003
+    heinzle = (3.72, 0.527, 0.53, 1.7)
+    _core.force_integrator(1 if integrator == "batched" else 0)
+    try:
+        for key, v in zip(["k1", "k2", "k3", "V_0"], heinzle):
+            _core.set_const(key, v)
+        np.testing.assert_allclose(run(), _closed_form(theta, *heinzle), rtol=1e-11, atol=1e-14)
+    finally:
+        for key, v in zip(["k1", "k2", "k3", "V_0"], friston):
+            _core.set_const(key, v)
+        _core.force_integrator(-1)
+    _core.force_integrator(1 if integrator == "batched" else 0)
+    try:
+        np.testing.assert_allclose(run(), _closed_form(theta, *friston), rtol=1e-11, atol=1e-14)
+    finally:
+        _core.force_integrator(-1)

@@ -5,9 +5,10 @@ with its device helpers utils.py:306-507, which follow the same formulas).
 Tolerances: Pearson r within 1e-12 absolute (two-pass FP64 in both); the KS
 statistic is a difference of two counts over sample sizes, so it must be
 equal to SciPy's to the last bit (1e-15 allows for its final division
-order).  Ties within and across samples, non-power-of-two sizes and NaN rows
-are covered; NaN gives NaN (callers map it to the worst score,
-sim/base.py:37-42).
+order).  Ties within and across samples, non-power-of-two sizes, NaN rows
+and sizes past one LDS sort chunk (8,192 values: C5's 74,305 FCD values per
+simulation, and a NaN in a later chunk) are covered; NaN gives NaN (callers
+map it to the worst score, sim/base.py:37-42).
 """
 import numpy as np
 import pytest
@@ -54,6 +55,27 @@ def test_fcd_ks_matches_scipy(core, n_sim_values, n_emp):
     emp = rng.uniform(-0.2, 1.0, n_emp)
     sim = rng.uniform(-0.5, 1.0, (9, n_sim_values)) + rng.uniform(-0.3, 0.3, (9, 1))
     np.testing.assert_allclose(core.gof_fcd_ks(sim, emp), _ks_ref(sim, emp), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("n_sim_values,n_emp", [(8192, 300), (8193, 1000), (74305, 74305), (74691, 300)])
+def test_fcd_ks_large_matches_scipy(core, n_sim_values, n_emp):
+    """Several sorted chunks per simulation (C5: 386-387 windows give
+    74,305-74,691 FCD values per simulation)."""
+    rng = np.random.default_rng(n_sim_values)
+    emp = np.tanh(rng.standard_normal(n_emp) * 0.5 + 0.3)
+    sim = np.tanh(rng.standard_normal((3, n_sim_values)) * 0.5 + rng.uniform(0, 0.6, (3, 1)))
+    sim[2, ::7] = np.round(sim[2, ::7], 2)  # ties across chunks
+    np.testing.assert_allclose(core.gof_fcd_ks(sim, emp), _ks_ref(sim, emp), rtol=0, atol=1e-15)
+
+
+def test_fcd_ks_large_nan_in_late_chunk(core):
+    rng = np.random.default_rng(11)
+    emp = rng.standard_normal(500)
+    sim = rng.standard_normal((2, 20000))
+    sim[1, 19999] = np.nan
+    ks = core.gof_fcd_ks(sim, emp)
+    assert not np.isnan(ks[0]) and np.isnan(ks[1])
+    assert ks[0] == pytest.approx(scipy.stats.ks_2samp(sim[0], emp).statistic, abs=1e-15)
 
 
 def test_fcd_ks_with_ties(core):

@@ -91,3 +91,43 @@ def test_pair_mode_requests_match_oracle():
     for r, g, tol in zip(reqs, grouped, [1e-9, 1e-7]):
         ref = cases.run_oracle(cases.copy_args(r))
         assert cases.bold_rel_err(g["sim_bold"], ref["sim_bold"]).max() <= tol, r[0]
+
+
+@pytest.mark.parametrize("pair", [1, -1], ids=["pair_mode", "single_column"])
+def test_c3_shaped_large_networks_match_oracle(pair):
+    """The uneven batch's large members: N = 200 and 400 (TR 0.72 s; the
+    Kuramoto request N = 200 at TR 0.5 s), every simulation with its own
+    seeded lognormal SC (the per-simulation integrator streams each
+    SC from memory), rWW, JR and Kuramoto, in one grouped call;
+    the coupling sums two adjacent SC columns per lane with 16-byte loads
+    (pair mode) or one column per lane (the form many one-SC-each
+    simulations take).  Kuramoto is chaotic and is checked at 1e-7 over a
+    short horizon, the others at 1e-9."""
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    # (model, N, duration s, simulations, seed, TR s); the oracle's Kuramoto
+    # evaluates N^2 sines per step, so that request is kept short
+    specs = [("rWW", 200, 5.0, 3, 21, 0.72), ("rWW", 400, 3.0, 2, 22, 0.72), ("JR", 200, 4.0, 2, 23, 0.72),
+             ("Kuramoto", 200, 1.5, 2, 24, 0.5)]
+    reqs = []
+    for model, N, dur, S, seed, TR in specs:
+        scs = np.stack([cases.synthetic_sc(N, seed=seed * 100 + i) for i in range(S)])
+        gp, rp = cases.params(model, S, N, G=np.linspace(0.2, 0.8, S), do_fic=False)
+        cfg = cases.model_config(model, TR=TR, window_size=2, window_step=1, bold_remove_s=0, do_fcd=False,
+                                 do_fic=False)
+        reqs.append(cases.sim_args(model, scs, gp, rp, cfg, S, duration_s=dur, TR_s=TR,
+                                   sc_indices=np.arange(S), sim_seed=seed % 4))
+    _core.core.set_tuning("general_pair", pair)
+    try:
+        grouped = _core.core.run_groups([cases.copy_args(r) for r in reqs])
+    finally:
+        _core.core.set_tuning("general_pair", 0)
+    for r, g, spec in zip(reqs, grouped, specs):
+        ref = cases.run_oracle(cases.copy_args(r))
+        tol = 1e-7 if spec[0] == "Kuramoto" else 1e-9
+        msg = f"{spec[0]} N={spec[1]}"
+        assert cases.bold_rel_err(g["sim_bold"], ref["sim_bold"]).max() <= tol, msg
+        assert cases.max_abs_err(g["sim_fc_trils"], ref["sim_fc_trils"]) <= tol, msg
+        s, rs = g["_sim_states"], ref["_sim_states"]
+        scale = np.maximum(np.nanmax(np.abs(rs), axis=(1, 2), keepdims=True), 1e-300)
+        assert cases.max_abs_err(s / scale, rs / scale) <= tol, msg
