@@ -51,6 +51,7 @@ def lib():
     L.cubnm_gof_fcd_ks.argtypes = [P, I, I, P, I, P]
     L.cubnm_gof_fc_corr_device.argtypes = [P, I, I, P, P]
     L.cubnm_gof_fcd_ks_device.argtypes = [P, I, I, P, I, P]
+    L.cubnm_gof_fcd_ks_device_static.argtypes = [P, I, I, P, I, P, P]
     LL = ctypes.POINTER(ctypes.c_longlong)
     L.cubnm_copy_noise.argtypes = [P, P, P]
     L.cubnm_rww_grad_sizes.argtypes = [I, I, I, I, D, D, I, LL, LL, LL]
@@ -107,6 +108,12 @@ def gof_fc_corr_device(sim_fc, n_sims, n_pairs, emp_fc, out):
 
 def gof_fcd_ks_device(sim_fcd, n_sims, n, emp_sorted, m, out):
     _check(lib().cubnm_gof_fcd_ks_device(sim_fcd, n_sims, n, emp_sorted, m, out))
+
+
+def gof_fcd_ks_device_static(sim_fcd, n_sims, n, emp_sorted, m, out, stream=None):
+    """Allocation-free KS (fixed per-GPU workspace); queued on ``stream``
+    (a hipStream_t address, None = null stream) without waiting."""
+    _check(lib().cubnm_gof_fcd_ks_device_static(sim_fcd, n_sims, n, emp_sorted, m, out, stream))
 
 
 def copy_noise(noise, shuffled_nodes, shuffled_ts):

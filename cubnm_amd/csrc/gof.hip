@@ -207,37 +207,75 @@ hipError_t fc_corr_launch(const double* d_sim, int n_sims, int n_pairs, const do
     return hipGetLastError();
 }
 
-// KS over all simulations, in batches whose sorted copies fit the workspace
-// (at most 256 MB or one simulation).
-hipError_t fcd_ks_launch(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m, double* d_out) {
+// KS over all simulations in batches of at most `batch`, with the sorted
+// copies in ws (batch * n doubles) and the per-simulation reductions in dmax
+// and nan_flag (batch each), on stream st; launches only, no allocation and
+// no synchronisation.
+hipError_t fcd_ks_run(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m, double* d_out,
+                      double* ws, unsigned long long* dmax, int* nan_flag, int batch, hipStream_t st) {
     using namespace cubnm;
-    GofBuffers& B = bufs();
     int chunk = 1;
     while (chunk < std::min(n, KS_CHUNK)) chunk <<= 1;
-    const size_t per_sim = sizeof(double) * (size_t)n;
-    // grid.y carries the batch: at most 65,535 simulations per launch
-    const int batch = (int)std::max<size_t>(
-        1, std::min<size_t>({(size_t)n_sims, ((size_t)256 << 20) / per_sim, (size_t)65535}));
-    hipError_t e;
-    if ((e = B.ensure(BUF_KS_WS, per_sim * batch)) != hipSuccess) return e;
-    if ((e = B.ensure(BUF_KS_RED, (sizeof(unsigned long long) + sizeof(int)) * batch)) != hipSuccess) return e;
-    double* ws = static_cast<double*>(B.p[BUF_KS_WS]);
-    unsigned long long* dmax = static_cast<unsigned long long*>(B.p[BUF_KS_RED]);
-    int* nan_flag = reinterpret_cast<int*>(dmax + batch);
     const int n_chunks = (n + chunk - 1) / chunk;
     const int per_block = GOF_THREADS * 16;
     const int eval_blocks = (n + m + per_block - 1) / per_block;
     for (int s0 = 0; s0 < n_sims; s0 += batch) {
         const int nb = std::min(batch, n_sims - s0);
-        hipLaunchKernelGGL(ks_clear_kernel, dim3((nb + 255) / 256), dim3(256), 0, 0, nb, dmax, nan_flag);
-        hipLaunchKernelGGL(ks_sort_chunks_kernel, dim3(n_chunks, nb), dim3(GOF_THREADS), sizeof(double) * chunk, 0,
+        hipLaunchKernelGGL(ks_clear_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, nb, dmax, nan_flag);
+        hipLaunchKernelGGL(ks_sort_chunks_kernel, dim3(n_chunks, nb), dim3(GOF_THREADS), sizeof(double) * chunk, st,
                            d_sim, s0, n, chunk, ws, nan_flag);
-        hipLaunchKernelGGL(ks_eval_kernel, dim3(eval_blocks, nb), dim3(GOF_THREADS), 0, 0, ws, n, chunk,
+        hipLaunchKernelGGL(ks_eval_kernel, dim3(eval_blocks, nb), dim3(GOF_THREADS), 0, st, ws, n, chunk,
                            d_emp_sorted, m, per_block, dmax);
-        hipLaunchKernelGGL(ks_finish_kernel, dim3((nb + 255) / 256), dim3(256), 0, 0, s0, nb, n, m, dmax, nan_flag,
+        hipLaunchKernelGGL(ks_finish_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, s0, nb, n, m, dmax, nan_flag,
                            d_out);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
+    return hipSuccess;
+}
+
+// Simulations per batch for a workspace of `bytes`: n doubles of sorted
+// copies plus 12 bytes of reductions each; at most 65,535 (grid.y).
+size_t ks_batch_for(size_t bytes, int n) {
+    return std::min<size_t>(bytes / (sizeof(double) * (size_t)n + sizeof(unsigned long long) + sizeof(int)), 65535);
+}
+
+// The same over the cached (grown on demand) device buffers.
+hipError_t fcd_ks_launch(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m, double* d_out) {
+    GofBuffers& B = bufs();
+    const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)n_sims, ks_batch_for((size_t)256 << 20, n)));
+    hipError_t e;
+    if ((e = B.ensure(BUF_KS_WS, sizeof(double) * (size_t)n * batch)) != hipSuccess) return e;
+    if ((e = B.ensure(BUF_KS_RED, (sizeof(unsigned long long) + sizeof(int)) * batch)) != hipSuccess) return e;
+    unsigned long long* dmax = static_cast<unsigned long long*>(B.p[BUF_KS_RED]);
+    return fcd_ks_run(d_sim, n_sims, n, d_emp_sorted, m, d_out, static_cast<double*>(B.p[BUF_KS_WS]), dmax,
+                      reinterpret_cast<int*>(dmax + batch), batch, 0);
+}
+
+// Fixed-size workspace of the allocation-free KS entry point, one per GPU.
+// Host code must not use this name as an address: it is resolved per device
+// with hipGetSymbolAddress (ks_static_scratch) and the result passed on.
+constexpr size_t KS_STATIC_BYTES = (size_t)16 << 20;
+__device__ __align__(256) unsigned char scratch[KS_STATIC_BYTES];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+// Device address of this GPU's scratch array, resolved on its first use on
+// the current device.
+hipError_t ks_static_scratch(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
     return hipSuccess;
 }
 
@@ -247,6 +285,8 @@ extern "C" {
 
 int cubnm_gof_fc_corr_device(const double* d_sim, int n_sims, int n_pairs, const double* d_emp, double* d_out);
 int cubnm_gof_fcd_ks_device(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m, double* d_out);
+int cubnm_gof_fcd_ks_device_static(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m,
+                                   double* d_out, void* stream);
 
 const char* cubnm_gof_last_error(void) { return g_gof_err.c_str(); }
 
@@ -264,6 +304,30 @@ int cubnm_gof_fcd_ks_device(const double* d_sim, int n_sims, int n, const double
     return guarded([&]() -> hipError_t {
         hipError_t e = fcd_ks_launch(d_sim, n_sims, n, d_emp_sorted, m, d_out);
         return e == hipSuccess ? hipStreamSynchronize(0) : e;
+    });
+}
+
+int cubnm_gof_fcd_ks_device_static(const double* d_sim, int n_sims, int n, const double* d_emp_sorted, int m,
+                                   double* d_out, void* stream) {
+    if (n_sims <= 0 || n <= 0 || m <= 0) { g_gof_err = "need n_sims, n and m > 0"; return -1; }
+    const size_t batch = ks_batch_for(KS_STATIC_BYTES, n);
+    if (batch < 1) {
+        g_gof_err = "cubnm_gof_fcd_ks_device_static: " + std::to_string(n) +
+                    " values per simulation exceed its fixed workspace of " + std::to_string(KS_STATIC_BYTES) +
+                    " bytes (at most " + std::to_string((KS_STATIC_BYTES - 12) / 8) + " values)";
+        cubnm::set_last_error(g_gof_err);
+        return -1;
+    }
+    return guarded([&]() -> hipError_t {
+        void* p = nullptr;
+        hipError_t e = ks_static_scratch(&p);
+        if (e != hipSuccess) return e;
+        const int nb = (int)std::min<size_t>(batch, (size_t)n_sims);
+        double* ws = static_cast<double*>(p);
+        unsigned long long* dmax = reinterpret_cast<unsigned long long*>(ws + (size_t)n * nb);
+        int* nan_flag = reinterpret_cast<int*>(dmax + nb);
+        return fcd_ks_run(d_sim, n_sims, n, d_emp_sorted, m, d_out, ws, dmax, nan_flag, nb,
+                          static_cast<hipStream_t>(stream));
     });
 }
 

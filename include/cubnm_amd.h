@@ -213,6 +213,21 @@ int cubnm_gof_fc_corr_device(const double* d_sim_fc, int n_sims, int n_pairs, co
 int cubnm_gof_fcd_ks_device(const double* d_sim_fcd, int n_sims, int n_sim_values,
                             const double* d_emp_fcd_sorted, int n_emp_values, double* d_out);
 
+/* Allocation-free variant of cubnm_gof_fcd_ks_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its sort and reduction workspace is a fixed 16 MiB array in the library's
+ * device image (one per GPU, held until the process ends), so it calls no
+ * hipMalloc/hipFree; simulations are processed in batches that fit it, and a
+ * simulation with more than 2,097,150 FCD values is rejected with an error.
+ * Work is queued on `stream` (a hipStream_t; NULL = the null stream) and the
+ * call returns without waiting.  Every call on one GPU shares that GPU's
+ * workspace: calls that overlap on one GPU (other streams, other threads)
+ * race and must be ordered by the caller. */
+int cubnm_gof_fcd_ks_device_static(const double* d_sim_fcd, int n_sims, int n_sim_values,
+                                   const double* d_emp_fcd_sorted, int n_emp_values, double* d_out,
+                                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif
