@@ -74,6 +74,9 @@ struct FcArgs {
 // allowed; see general_pair_ok)
 hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* model_consts, hipStream_t s,
                                     bool latency_bound);
+// more than 1024 nodes (up to 8192): one 1024-lane workgroup per simulation,
+// several nodes per lane; no delay, no numerical-FIC restarts
+hipError_t launch_integrate_wide(int model, const IntegrateArgs& a, const void* model_consts, hipStream_t s);
 // general kernel geometry: lanes per k partition, partitions (want 0 = auto),
 // LDS bytes of a workgroup; several requests of one model in one launch
 int general_np(int nodes);

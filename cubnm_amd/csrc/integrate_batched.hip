@@ -379,6 +379,7 @@ hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const voi
         case MODEL_JR: return dispatch<JR>(ba, consts, tpw, s);
         case MODEL_KURAMOTO: return dispatch<Kuramoto>(ba, consts, tpw, s);
         case MODEL_WC: return dispatch<WC>(ba, consts, tpw, s);
+        case MODEL_RWWEX: return dispatch<RWWEx>(ba, consts, tpw, s);
         default: return hipErrorInvalidValue;
     }
 }

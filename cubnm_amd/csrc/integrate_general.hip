@@ -3,7 +3,8 @@
 // This kernel covers every feature of the reference simulation loop
 // (src/ext/bnm.cu:117-505, semantics of src/ext/bnm.cpp:69-522): conduction
 // delays with a circular history, oscillator coupling, rWW numerical-FIC
-// restarts, state time series, any node count up to 1024.  The batched
+// restarts, state time series, any node count up to 1024 (and, in
+// integrate_wide_kernel below, up to 8192 nodes).  The batched
 // MFMA kernel (integrate_batched.hip) is the throughput path for the common
 // case (no delay, shared SC); this one is the feature-complete path.
 //
@@ -371,6 +372,121 @@ __global__ void __launch_bounds__(1024) integrate_general_kernel(IntegrateArgs a
     general_body<M>(a, mc, blockIdx.x, nullptr);
 }
 
+// More than 1024 nodes: one workgroup of 1024 lanes per simulation, lane t
+// owning nodes t, t + 1024, ..., t + (NPL - 1) * 1024; the coupling state
+// (N doubles, sin and cos for oscillators) in LDS and SC read from memory.
+// This is the reference's many-node regime (bnm.cu:702-743 splits such a
+// simulation over co-launched blocks with a grid barrier per step); here one
+// CU integrates one simulation and simulations run side by side, so no
+// grid-wide barrier is needed.  Without conduction delay and numerical-FIC
+// restarts (the host rejects both for N > 1024); per node the arithmetic and
+// summation order are those of integrate_general_kernel with one partition.
+template <class M, int NPL>
+__global__ void __launch_bounds__(1024) integrate_wide_kernel(IntegrateArgs a, typename M::C mc) {
+    const int sim = blockIdx.x;
+    const int N = a.nodes, L = blockDim.x;
+    extern __shared__ double conn[];  // [N], oscillators [2N]
+    const gmem<const double>* SCg = as_global(a.SC) + (long long)a.SC_indices[sim] * N * N;
+    double gp[M::n_global];
+    for (int i = 0; i < M::n_global; i++) gp[i] = a.gp[(long long)i * a.N_SIMS + sim];
+    double rp[NPL][M::n_regional], s[NPL][M::n_state], st_sum[NPL][M::n_state];
+    double bx[NPL], bf[NPL], bn[NPL], bq[NPL];
+    int jj[NPL];
+    bool act[NPL];
+#pragma unroll
+    for (int v = 0; v < NPL; v++) {
+        const int j = threadIdx.x + v * L;
+        act[v] = j < N;
+        jj[v] = act[v] ? j : 0;
+        for (int i = 0; i < M::n_regional; i++)
+            rp[v][i] = a.rp[(long long)i * a.N_SIMS * N + (long long)sim * N + jj[v]];
+        for (int i = 0; i < M::n_state; i++) { s[v][i] = 0.0; st_sum[v][i] = 0.0; }
+        M::init(mc, s[v], rp[v]);
+        bx[v] = 0.0; bf[v] = 1.0; bn[v] = 1.0; bq[v] = 1.0;
+    }
+    auto publish = [&](int j, double val) {
+        if constexpr (M::is_osc) sincos(val, &conn[j], &conn[N + j]);
+        else conn[j] = val;
+    };
+#pragma unroll
+    for (int v = 0; v < NPL; v++)
+        if (act[v]) publish(jj[v], s[v][M::conn_idx]);
+    const long long st_stride = a.states_ts ? (long long)a.states_len * N : N;
+    gmem<double>* bold = as_global(a.bold) + (long long)sim * a.bold_len * N;
+    const gmem<const double>* noise = as_global(a.noise);
+    int bw_i = 0, bold_i = 0, states_i = 0, rep = 0;
+    while (bw_i < a.bw_it) {
+        const int sh_t = as_global(a.sh_ts)[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it];
+        int sh_j[NPL];
+#pragma unroll
+        for (int v = 0; v < NPL; v++) sh_j[v] = as_global(a.sh_nodes)[rep * N + jj[v]];
+        for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
+            double n0[NPL], n1[NPL];
+#pragma unroll
+            for (int v = 0; v < NPL; v++) {
+                const long long nidx = ((long long)(sh_t * a.inner_it + inner_i) * N + sh_j[v]) * M::n_noise;
+                n0[v] = act[v] ? noise[nidx] : 0.0;
+                n1[v] = (M::n_noise > 1 && act[v]) ? noise[nidx + 1] : 0.0;
+            }
+            lds_barrier();
+#pragma unroll
+            for (int v = 0; v < NPL; v++) {
+                if (!act[v]) continue;
+                double g = 0.0, g2 = 0.0;
+                coupling_sum<M::is_osc>(SCg + jj[v], N, conn, N, 0, N, g, g2);
+                if constexpr (M::is_osc) g = conn[N + jj[v]] * g - conn[jj[v]] * g2;
+                M::step(mc, s[v], gp, rp[v], g, n0[v], n1[v]);
+            }
+            lds_barrier();
+#pragma unroll
+            for (int v = 0; v < NPL; v++)
+                if (act[v]) publish(jj[v], s[v][M::conn_idx]);
+        }
+#pragma unroll
+        for (int v = 0; v < NPL; v++)
+            if (act[v]) bw_step(a.bw, s[v][M::bold_idx], bx[v], bf[v], bn[v], bq[v]);
+        if ((bw_i + 1) % a.BOLD_TR_iters == 0) {
+            if (bold_i < a.bold_len)
+#pragma unroll
+                for (int v = 0; v < NPL; v++)
+                    if (act[v]) bold[(long long)bold_i * N + jj[v]] = bold_signal(a.bw, bn[v], bq[v]);
+            bold_i++;
+        }
+        if ((bw_i + 1) % a.states_sampling_iters == 0) {
+#pragma unroll
+            for (int v = 0; v < NPL; v++) {
+                if (!act[v]) continue;
+                if (a.states_ts) {
+                    if (states_i < a.states_len)
+                        for (int i = 0; i < M::n_state; i++)
+                            a.states[(long long)i * a.N_SIMS * st_stride + (long long)sim * st_stride +
+                                     (long long)states_i * N + jj[v]] = s[v][i];
+                } else if (states_i >= a.n_states_samples_remove) {
+                    for (int i = 0; i < M::n_state; i++) st_sum[v][i] += s[v][i];
+                }
+            }
+            states_i++;
+        }
+        if ((bw_i + 1) % a.noise_bw_it == 0 && bw_i + 1 < a.bw_it) rep++;
+        bw_i++;
+    }
+#pragma unroll
+    for (int v = 0; v < NPL; v++) {
+        if (!act[v]) continue;
+        const int j = jj[v];
+        if constexpr (M::n_gout_int > 0) {
+            // analytical FIC only (restarts are rejected for this kernel):
+            // wIE is unchanged, no trials, no failure
+            if (mc.do_fic && j == 0) { a.gint[sim] = 0; a.gbool[1LL * a.N_SIMS + sim] = false; }
+        }
+        if (!a.states_ts) {
+            const int tp = states_i - a.n_states_samples_remove;
+            for (int i = 0; i < M::n_state; i++)
+                a.states[(long long)i * a.N_SIMS * st_stride + (long long)sim * st_stride + j] = st_sum[v][i] / tp;
+        }
+    }
+}
+
 // Every per-simulation request of a heterogeneous batch in one launch: block b
 // integrates simulation blocks[b].y of request blocks[b].x, with that
 // request's model (models[r]), arguments and model constants (consts + r *
@@ -389,6 +505,7 @@ __global__ void __launch_bounds__(1024) integrate_general_multi_kernel(const Int
         case MODEL_KURAMOTO:
             general_body<Kuramoto>(args[b.x], *reinterpret_cast<const Kuramoto::C*>(c), b.y, progress);
             break;
+        case MODEL_RWWEX: general_body<RWWEx>(args[b.x], *reinterpret_cast<const RWWEx::C*>(c), b.y, progress); break;
         default: general_body<WC>(args[b.x], *reinterpret_cast<const WC::C*>(c), b.y, progress); break;
     }
     if (threadIdx.x == 0 && progress)
@@ -464,6 +581,7 @@ size_t model_consts_bytes(int model) {
         case MODEL_RWW: return sizeof(RWW::C);
         case MODEL_JR: return sizeof(JR::C);
         case MODEL_KURAMOTO: return sizeof(Kuramoto::C);
+        case MODEL_RWWEX: return sizeof(RWWEx::C);
         default: return sizeof(WC::C);
     }
 }
@@ -472,7 +590,7 @@ size_t model_consts_bytes(int model) {
 // constants, rounded up to 16 bytes.
 size_t model_consts_stride() {
     size_t m = 0;
-    for (int k : {MODEL_RWW, MODEL_JR, MODEL_KURAMOTO, MODEL_WC}) m = std::max(m, model_consts_bytes(k));
+    for (int k : {MODEL_RWW, MODEL_JR, MODEL_KURAMOTO, MODEL_WC, MODEL_RWWEX}) m = std::max(m, model_consts_bytes(k));
     return (m + 15) & ~(size_t)15;
 }
 
@@ -490,6 +608,31 @@ hipError_t launch_integrate_general_multi(const IntegrateArgs* d_args, const voi
     return hipGetLastError();
 }
 
+template <class M>
+static hipError_t launch_wide(const IntegrateArgs& a, const void* consts, hipStream_t st) {
+    const typename M::C mc = *static_cast<const typename M::C*>(consts);
+    if (a.do_delay) return hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * (size_t)(M::is_osc ? 2 : 1) * a.nodes;
+    const int npl = (a.nodes + 1023) / 1024;
+    if (npl <= 2) hipLaunchKernelGGL((integrate_wide_kernel<M, 2>), dim3(a.N_SIMS), dim3(1024), lds, st, a, mc);
+    else if (npl <= 4) hipLaunchKernelGGL((integrate_wide_kernel<M, 4>), dim3(a.N_SIMS), dim3(1024), lds, st, a, mc);
+    else if (npl <= 8) hipLaunchKernelGGL((integrate_wide_kernel<M, 8>), dim3(a.N_SIMS), dim3(1024), lds, st, a, mc);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_wide(int model, const IntegrateArgs& a, const void* consts, hipStream_t s) {
+    if (a.nodes <= 1024 || a.nodes > 8192) return hipErrorInvalidValue;
+    switch (model) {
+        case MODEL_RWW: return launch_wide<RWW>(a, consts, s);
+        case MODEL_JR: return launch_wide<JR>(a, consts, s);
+        case MODEL_KURAMOTO: return launch_wide<Kuramoto>(a, consts, s);
+        case MODEL_WC: return launch_wide<WC>(a, consts, s);
+        case MODEL_RWWEX: return launch_wide<RWWEx>(a, consts, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
                                     bool latency_bound) {
     if (a.nodes < 1 || a.nodes > 1024) return hipErrorInvalidValue;
@@ -498,6 +641,7 @@ hipError_t launch_integrate_general(int model, const IntegrateArgs& a, const voi
         case MODEL_JR: return launch_general<JR>(model, a, consts, s, latency_bound);
         case MODEL_KURAMOTO: return launch_general<Kuramoto>(model, a, consts, s, latency_bound);
         case MODEL_WC: return launch_general<WC>(model, a, consts, s, latency_bound);
+        case MODEL_RWWEX: return launch_general<RWWEx>(model, a, consts, s, latency_bound);
         default: return hipErrorInvalidValue;
     }
 }

@@ -6,6 +6,7 @@
 //   JR        include/cubnm/models/jr.hpp:55-75,   src/ext/models/jr.cpp:14-133
 //   Kuramoto  include/cubnm/models/kuramoto.hpp,   src/ext/models/kuramoto.cpp:14-61
 //   WC        include/cubnm/models/wc.hpp:50-70,   src/ext/models/wc.cpp:14-80
+//   rWWEx     include/cubnm/models/rwwex.hpp,      src/ext/models/rwwex.cpp:14-66
 // The expressions keep the reference's operation order (left-to-right
 // products such as (w_p*J_N)*S_E) so that FP64 results agree with the CPU
 // path to rounding; noise enters as already-gathered values.
@@ -24,7 +25,7 @@
 
 namespace cubnm {
 
-enum ModelId { MODEL_RWW = 0, MODEL_JR = 1, MODEL_KURAMOTO = 2, MODEL_WC = 3, MODEL_COUNT = 4 };
+enum ModelId { MODEL_RWW = 0, MODEL_JR = 1, MODEL_KURAMOTO = 2, MODEL_WC = 3, MODEL_RWWEX = 4, MODEL_COUNT = 5 };
 
 // Balloon-Windkessel constants (src/ext/models/bw.cpp:4-22; core.cpp:157-192, 291-292)
 struct BWConsts {
@@ -246,6 +247,47 @@ struct WC {
         const double dE = (-s[0] + (1.0 - s[0]) * SE) * c.itau_E;
         s[0] += c.dt * dE + c.sqrt_dt * n0 * rp[6];
         s[0] = fmax(0.0, fmin(1.0, s[0]));
+    }
+};
+
+// reduced Wong-Wang, excitatory population only ------------------------------
+// State [x, r, S]; regional [w, I0, sigma]; global [G]; one noise per node.
+struct RWWEx {
+    static constexpr int n_state = 3, n_noise = 1, n_global = 1, n_regional = 3;
+    static constexpr int conn_idx = 2, bold_idx = 2, n_gout_bool = 0, n_gout_int = 0;
+    static constexpr bool is_osc = false;
+    struct C { double dt, sqrt_dt, J_N, a, b, d, dt_itau, dt_gamma; };
+    static void init_constants(C& c, double dt) {
+        c.dt = dt; c.sqrt_dt = std::sqrt(dt);
+        c.J_N = 0.2609; c.a = 270; c.b = 108; c.d = 0.154;
+        const double gamma = (double)0.641 / (double)1000.0, tau = 100;
+        c.dt_itau = dt * (1.0 / tau); c.dt_gamma = dt * gamma;
+    }
+    CUBNM_HD static void init(const C&, double* s, const double*) { s[2] = 0.001; }
+    // rwwex.cpp:45-66
+    CUBNM_HD static void step(const C& c, double* s, const double* gp, const double* rp, double g,
+                              double n0, double) {
+        s[0] = rp[0] * c.J_N * s[2] + gp[0] * c.J_N * g + rp[1];
+        const double axb = c.a * s[0] - c.b;
+        s[1] = axb / (1 - exp(-c.d * axb));
+        const double dS = c.dt_gamma * ((1 - s[2]) * s[1]) - c.dt_itau * s[2] + n0 * c.sqrt_dt * rp[2];
+        s[2] += dS;
+        s[2] = fmax(0.0, fmin(1.0, s[2]));
+    }
+    // `step` split around the coupling input (see RWW::pre): only w J_N S
+    // precedes it.
+    static constexpr int n_tmp = 1;
+    CUBNM_HD static void pre(const C& c, double* s, const double*, const double* rp, double, double, double* t) {
+        t[0] = rp[0] * c.J_N * s[2];
+    }
+    CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
+                              double, const double* t) {
+        s[0] = t[0] + gp[0] * c.J_N * g + rp[1];
+        const double axb = c.a * s[0] - c.b;
+        s[1] = DIV(axb, 1 - exp(-c.d * axb));
+        const double dS = c.dt_gamma * ((1 - s[2]) * s[1]) - c.dt_itau * s[2] + n0 * c.sqrt_dt * rp[2];
+        s[2] += dS;
+        s[2] = fmax(0.0, fmin(1.0, s[2]));
     }
 };
 

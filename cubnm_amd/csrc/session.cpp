@@ -88,6 +88,7 @@ const ModelInfo kModels[] = {
     {"JR", MODEL_JR, JR::n_state, JR::n_global, JR::n_regional, JR::n_noise, JR::n_gout_bool, JR::n_gout_int},
     {"Kuramoto", MODEL_KURAMOTO, Kuramoto::n_state, Kuramoto::n_global, Kuramoto::n_regional, Kuramoto::n_noise, 0, 0},
     {"WC", MODEL_WC, WC::n_state, WC::n_global, WC::n_regional, WC::n_noise, 0, 0},
+    {"rWWEx", MODEL_RWWEX, RWWEx::n_state, RWWEx::n_global, RWWEx::n_regional, RWWEx::n_noise, 0, 0},
 };
 
 const ModelInfo* find_model(const char* name) {
@@ -147,7 +148,7 @@ struct Session {
     double dt = 0.1, bw_dt = 0.001;  // ms, s
     BaseConf conf;
     bool modifies_params = false;
-    RWW::C rww; JR::C jr; Kuramoto::C kur; WC::C wc;
+    RWW::C rww; JR::C jr; Kuramoto::C kur; WC::C wc; RWWEx::C rwwex;
     // derived
     int bold_len = 0, bold_size = 0, BOLD_TR_iters = 0, states_len = 0, states_sampling_iters = 0;
     int bw_it = 0, inner_it = 0, n_states_samples_remove = 0, n_vols_remove = 0, corr_len = 0;
@@ -283,6 +284,7 @@ void init_model_constants(Session& s) {
     JR::init_constants(s.jr, s.dt);
     Kuramoto::init_constants(s.kur, s.dt);
     WC::init_constants(s.wc, s.dt);
+    RWWEx::init_constants(s.rwwex, s.dt);
 }
 
 const void* model_consts(const Session& s) {
@@ -290,6 +292,7 @@ const void* model_consts(const Session& s) {
         case MODEL_RWW: return &s.rww;
         case MODEL_JR: return &s.jr;
         case MODEL_KURAMOTO: return &s.kur;
+        case MODEL_RWWEX: return &s.rwwex;
         default: return &s.wc;
     }
 }
@@ -593,7 +596,14 @@ IntegrateArgs setup_integrate(Session& s, const DevIO& io, const std::vector<int
     if (g_force_integrator == 0) use_batched = false;
     if (g_force_integrator == 1 && !use_batched)
         throw std::runtime_error("batched integrator does not support this configuration (delay, numerical FIC, states_ts or node count)");
-    s.last_integrator = use_batched ? 1 : 0;
+    if (N > 1024) {
+        // the many-node integrator (integrate_wide_kernel)
+        if (any_delay) throw std::runtime_error("conduction delay is not supported with more than 1024 nodes");
+        if (fic_restarts)
+            throw std::runtime_error("numerical FIC (max_fic_trials > 0) is not supported with more than 1024 nodes");
+        use_batched = false;
+    }
+    s.last_integrator = use_batched ? 1 : (N > 1024 ? 2 : 0);
     s.n_groups = 0;
     if (use_batched) {
         s.n_groups = build_groups(s, SCi_host);
@@ -608,7 +618,9 @@ IntegrateArgs setup_integrate(Session& s, const DevIO& io, const std::vector<int
 void launch_integrator(Session& s, const IntegrateArgs& a) {
     int launches = 1;
     check(hipEventRecord(s.ev[2], s.stream), "event");
-    if (s.last_integrator == 1)
+    if (s.last_integrator == 2)
+        check(launch_integrate_wide(s.model->id, a, model_consts(s), s.stream), "integrate_wide launch");
+    else if (s.last_integrator == 1)
         check(launch_integrate_batched(s.model->id, a, model_consts(s), s.stream, &launches, s.d_groups.as<int>(),
                                        s.n_groups, s.d_afrag.as<double>(), s.tpw),
               "integrate_batched launch");
@@ -688,7 +700,7 @@ void prepare_session(Session& s, const char* model_name, int n_config, const cha
     const ModelInfo* m = find_model(model_name);
     if (!m) throw std::runtime_error(std::string("Model ") + model_name + " not found");
     if (N_SIMS <= 0 || nodes <= 0 || N_SCs <= 0) throw std::runtime_error("N_SIMS, nodes and N_SCs must be positive");
-    if (nodes > 1024) throw std::runtime_error("nodes > 1024 is not supported");
+    if (nodes > 8192) throw std::runtime_error("nodes > 8192 is not supported");
     if (dt <= 0 || bw_dt <= 0 || BOLD_TR <= 0 || states_sampling <= 0 || time_steps <= 0)
         throw std::runtime_error("durations and steps must be positive");
     const double t0 = now_s();
@@ -1068,7 +1080,7 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
         for (int k = 0; k < n_groups; k++) {
             const int i = order[k];
             Session& s = *g_pool[i];
-            if (s.last_integrator == 1) {
+            if (s.last_integrator == 1 || s.last_integrator == 2) {
                 launch_integrator(s, s.iargs);
                 batched.push_back(i);
                 continue;

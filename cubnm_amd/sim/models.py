@@ -1,5 +1,5 @@
 """Model SimGroups: parameter names, state names, noise width and defaults
-of each model (reference sim/rww.py, jr.py, kuramoto.py, wc.py), plus the
+of each model (reference sim/rww.py, jr.py, kuramoto.py, wc.py, rwwex.py), plus the
 rWW feedback-inhibition-control (FIC) options, outputs and penalty
 (rww.py:19-234) and the Kuramoto initial phases (kuramoto.py:28-137)."""
 import logging
@@ -166,3 +166,16 @@ class WCSimGroup(SimGroup):
     n_noise = 2
     default_params = {"G": None, "c_EE": 16, "c_IE": 12, "c_EI": 15, "c_II": 3, "P_E": 0.5, "P_I": 0.0,
                       "sigma_E": 5e-05, "sigma_I": 5e-05}
+
+
+class rWWExSimGroup(SimGroup):
+    """Reduced Wong-Wang model, excitatory population only (Deco et al. 2013):
+    x = w J_N S + G J_N sum_j C_ij S_j + I0, r = (a x - b) / (1 - exp(-d (a x - b))),
+    dS/dt = -S/tau + (1 - S) gamma r + sigma noise (sim/rwwex.py:14-66)."""
+    model_name = "rWWEx"
+    global_param_names = ["G"]
+    regional_param_names = ["w", "I0", "sigma"]
+    state_names = ["x", "r", "S"]
+    sel_state_var = "r"
+    n_noise = 1
+    default_params = {"G": None, "w": 0.9, "I0": 0.3, "sigma": 0.001}

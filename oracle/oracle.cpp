@@ -16,6 +16,7 @@
 //   * rWW constants/config/step/FIC ....... src/ext/models/rww.cpp:16-313
 //   * analytical FIC ...................... src/ext/models/rww_fic.cpp:105-231
 //   * JR / Kuramoto / WC steps ............ src/ext/models/jr.cpp:14-133, kuramoto.cpp:14-61, wc.cpp:14-80
+//   * rWWEx step .......................... src/ext/models/rwwex.cpp:14-66
 //   * FC / FCD trils ...................... src/ext/fc.cpp:1-91
 //   * pair count and FCD windows .......... src/ext/utils.cpp:6-72
 //
@@ -307,6 +308,30 @@ struct WC {
         s[1] += dt * iv[3] + sqrt_dt * noise[idx + 1] * rp[7];
         s[0] = std::max(0.0, std::min(1.0, s[0]));
         s[1] = std::max(0.0, std::min(1.0, s[1]));
+    }
+};
+
+struct RWWEx {
+    static const int n_state = 3, n_inter = 2, n_noise = 1, n_global = 1, n_regional = 3;
+    static const int conn_idx = 2, bold_idx = 2, n_gout_bool = 0, n_gout_int = 0;
+    static const bool is_osc = false;
+    double dt, sqrt_dt, J_N, a, b, d, gamma, tau, itau, dt_itau, dt_gamma;
+    void init_constants(double dt_) {
+        dt = dt_; sqrt_dt = std::sqrt(dt);
+        J_N = 0.2609; a = 270; b = 108; d = 0.154;
+        gamma = (double)0.641 / (double)1000.0; tau = 100; itau = 1.0 / tau;
+        dt_itau = dt * itau; dt_gamma = dt * gamma;
+    }
+    void init(double* s) const { s[0] = 0.0; s[1] = 0.0; s[2] = 0.001; }
+    // rwwex.cpp:45-66: state [x, r, S], regional [w, I0, sigma]
+    void step(double* s, double* iv, const double* gp, const double* rp, double g,
+              const double* noise, long idx) const {
+        s[0] = rp[0] * J_N * s[2] + gp[0] * J_N * g + rp[1];
+        iv[0] = a * s[0] - b;
+        s[1] = iv[0] / (1 - std::exp(-d * iv[0]));
+        iv[1] = dt_gamma * ((1 - s[2]) * s[1]) - dt_itau * s[2] + noise[idx] * sqrt_dt * rp[2];
+        s[2] += iv[1];
+        s[2] = std::max(0.0, std::min(1.0, s[2]));
     }
 };
 
@@ -681,10 +706,12 @@ static void init_node(const RWW& m, double* s, double* iv, double*, int* ei, boo
 static void init_node(const JR& m, double* s, double*, double*, int*, bool*) { m.init(s); }
 static void init_node(const Kuramoto& m, double* s, double*, double* rp, int*, bool*) { m.init(s, rp); }
 static void init_node(const WC& m, double* s, double*, double*, int*, bool*) { m.init(s); }
+static void init_node(const RWWEx& m, double* s, double*, double*, int*, bool*) { m.init(s); }
 static void restart_node(const RWW& m, double* s, double* iv, double*) { m.restart(s, iv); }
 static void restart_node(const JR& m, double* s, double*, double*) { m.init(s); }
 static void restart_node(const Kuramoto& m, double* s, double*, double* rp) { m.init(s, rp); }
 static void restart_node(const WC& m, double* s, double*, double*) { m.init(s); }
+static void restart_node(const RWWEx& m, double* s, double*, double*) { m.init(s); }
 
 // rww.cpp:259-291
 static void post_bw(const RWW& m, std::vector<double>& st, std::vector<double>& iv, std::vector<double>& rp,
@@ -781,6 +808,7 @@ static int model_counts(const std::string& name, int* n_state, int* n_global, in
     if (name == "JR") OR_COUNTS(JR)
     if (name == "Kuramoto") OR_COUNTS(Kuramoto)
     if (name == "WC") OR_COUNTS(WC)
+    if (name == "rWWEx") OR_COUNTS(RWWEx)
 #undef OR_COUNTS
     return -1;
 }
@@ -842,6 +870,9 @@ int oracle_run(const char* model, const char* config, int N_SIMS, int nodes, int
             run_model(m, t, c, d, bw, in, out, N_SCs, sim_seed, n_threads, noise_out, shn_out, sht_out, times);
         } else if (name == "WC") {
             WC m; compute_dims(t, c, WC::n_noise, d);
+            run_model(m, t, c, d, bw, in, out, N_SCs, sim_seed, n_threads, noise_out, shn_out, sht_out, times);
+        } else if (name == "rWWEx") {
+            RWWEx m; compute_dims(t, c, RWWEx::n_noise, d);
             run_model(m, t, c, d, bw, in, out, N_SCs, sim_seed, n_threads, noise_out, shn_out, sht_out, times);
         } else {
             g_err = "unknown model " + name;

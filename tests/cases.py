@@ -24,6 +24,7 @@ MODELS = {
     "WC": (["G"], ["c_EE", "c_IE", "c_EI", "c_II", "P_E", "P_I", "sigma_E", "sigma_I"],
            {"c_EE": 16, "c_IE": 12, "c_EI": 15, "c_II": 3, "P_E": 0.5, "P_I": 0.0,
             "sigma_E": 5e-05, "sigma_I": 5e-05}, ["E", "I"], 2),
+    "rWWEx": (["G"], ["w", "I0", "sigma"], {"w": 0.9, "I0": 0.3, "sigma": 0.001}, ["x", "r", "S"], 1),
 }
 
 FRISTON2003 = (7 * 0.34, 2.0, 2 * 0.34 - 0.2, 0.02 * 100)

@@ -151,7 +151,7 @@ def decode(data):
 
 def main(ref_root):
     here = os.path.dirname(os.path.abspath(__file__))
-    for model in ["rWW", "JR", "Kuramoto", "WC"]:
+    for model in ["rWW", "JR", "Kuramoto", "WC", "rWWEx"]:
         path = os.path.join(ref_root, "tests", "expected", "sim", f"{model}.pkl.gz")
         with gzip.open(path, "rb") as f:
             data = f.read()

@@ -14,6 +14,8 @@
   / rho.  The core must give that value under the heinzle2016-3T constants
   and a changed V_0 set through set_const (core.cpp:157-192), and the
   default constants again after set_const restores them.
+* More than 1024 nodes: the many-node integrator against the oracle, and its
+  refusal of conduction delays.
 """
 import numpy as np
 import pytest
@@ -103,3 +105,44 @@ def test_set_const_changes_bold_constants(integrator):
         np.testing.assert_allclose(run(), _closed_form(theta, *friston), rtol=1e-11, atol=1e-14)
     finally:
         _core.force_integrator(-1)
+
+
+@pytest.mark.parametrize("model,nodes", [("rWW", 1100), ("rWWEx", 2100), ("Kuramoto", 1300)])
+def test_many_nodes_match_oracle(model, nodes):
+    """More than 1024 nodes (the reference's co-launch regime, bnm.cu:702-743):
+    one 1024-lane workgroup per simulation with 2 or 4 nodes per lane, SC read
+    from memory.  BOLD and FC against the oracle (Kuramoto at 1e-7: chaotic)."""
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    sc = cases.synthetic_sc(nodes, seed=nodes)
+    S = 2 if nodes < 2000 else 1
+    # Kuramoto: with ~1300 inputs per node, G of order 0.1 makes the phase
+    # dynamics chaotic at dt 1 ms (rounding differences grow to O(1) within
+    # 2 s); G of order 1e-3 keeps it in a regime where they stay at rounding
+    G = np.linspace(5e-4, 1e-3, S) if model == "Kuramoto" else np.linspace(0.3, 0.6, S)
+    gp, rp = cases.params(model, S, nodes, G=G, do_fic=False)
+    cfg = cases.model_config(model, TR=0.5, window_size=2, window_step=1, bold_remove_s=0, do_fcd=False,
+                             do_fic=False)
+    args = cases.sim_args(model, sc, gp, rp, cfg, S, duration_s=2, TR_s=0.5, dt="1.0")
+    out = _core.run_simulations(*cases.copy_args(args))
+    assert _core.core.last_timings()["integrator"] == 2
+    ref = cases.run_oracle(cases.copy_args(args))
+    tol = 1e-7 if model == "Kuramoto" else TOL
+    assert cases.bold_rel_err(out["sim_bold"], ref["sim_bold"]).max() <= tol
+    assert cases.max_abs_err(out["sim_fc_trils"], ref["sim_fc_trils"]) <= tol
+    s, r = out["_sim_states"], ref["_sim_states"]
+    scale = np.maximum(np.nanmax(np.abs(r), axis=(1, 2), keepdims=True), 1e-300)
+    assert cases.max_abs_err(s / scale, r / scale) <= tol
+
+
+def test_many_nodes_rejects_delay():
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    N = 1030
+    gp, rp = cases.params("rWW", 1, N, G=[0.5], do_fic=False)
+    cfg = cases.model_config("rWW", TR=0.5, window_size=2, window_step=1, bold_remove_s=0, do_fcd=False,
+                             do_fic=False)
+    args = cases.sim_args("rWW", cases.synthetic_sc(N), gp, rp, cfg, 1, duration_s=1, TR_s=0.5, dt="1.0",
+                          sc_dist=cases.synthetic_sc(N, seed=3) * 50, v=1.0)
+    with pytest.raises(RuntimeError, match="more than 1024 nodes"):
+        _core.run_simulations(*args)

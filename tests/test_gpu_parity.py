@@ -60,7 +60,7 @@ def _run_both(core, args):
     return out, ref, a1, a2
 
 
-@pytest.mark.parametrize("model", ["rWW", "JR", "WC"])
+@pytest.mark.parametrize("model", ["rWW", "JR", "WC", "rWWEx"])
 def test_small_network_all_models(core, model):
     N, S = 10, 3
     sc = cases.synthetic_sc(N, seed=1)
@@ -169,6 +169,17 @@ def test_batched_node_counts(core, nodes):
     args = cases.sim_args("rWW", sc, gp, rp, cfg, S, duration_s=10)
     out, ref, _, _ = _run_both(core, args)
     _compare(out, ref, "rWW")
+
+
+def test_rwwex_c1_shape(core):
+    """rWWEx on the C1 shape (N = 100, 60 s, 8 simulations over G)."""
+    N = 100
+    sc = cases.synthetic_sc(N, seed=0)
+    gp, rp = cases.params("rWWEx", 8, N, G=np.linspace(0.2, 2.0, 8))
+    cfg = cases.model_config("rWWEx")
+    args = cases.sim_args("rWWEx", sc, gp, rp, cfg, 8, duration_s=60)
+    out, ref, _, _ = _run_both(core, args)
+    _compare(out, ref, "rWWEx")
 
 
 def test_batched_mixed_scs(core):

@@ -24,11 +24,11 @@ def _group(model_cls, **kw):
     return sg
 
 
-@pytest.mark.parametrize("model", ["rWW", "JR", "WC"])
+@pytest.mark.parametrize("model", ["rWW", "JR", "WC", "rWWEx"])
 def test_simgroup_gpu_vs_oracle(model, monkeypatch):
     import cubnm_amd.sim as sim
     from cubnm_amd import _core
-    cls = {"rWW": sim.rWWSimGroup, "JR": sim.JRSimGroup, "WC": sim.WCSimGroup}[model]
+    cls = {"rWW": sim.rWWSimGroup, "JR": sim.JRSimGroup, "WC": sim.WCSimGroup, "rWWEx": sim.rWWExSimGroup}[model]
     gpu = _group(cls)
     gpu.run()
     ref = _group(cls)

@@ -5,7 +5,7 @@
    system per node, with the coupling input summed from the previous step's
    states (bnm.cpp:69-522).  The right-hand sides below are written from the
    reference's model sources (rww.cpp:187-225, jr.cpp:80-133, wc.cpp:51-80,
-   kuramoto.cpp:48-61) as continuous derivatives, and integrated by
+   kuramoto.cpp:48-61, rwwex.cpp:45-66) as continuous derivatives, and integrated by
    scipy's DOP853 at rtol 1e-12 on a three-node network with an asymmetric
    SC (so the coupling orientation SC[k][j], source k -> target j, matters)
    and G > 0.  The oracle's state time series (states_ts, sampled every 1 ms)
@@ -86,6 +86,17 @@ def wc_rhs(G, c_EE=16.0, c_IE=12.0, c_EI=15.0, c_II=3.0, P_E=0.5, P_I=0.0):
     return f, np.full(6, 0.001), [0, 1]
 
 
+def rwwex_rhs(G, w=0.9, I0=0.3):
+    J_N, a, b, d, gamma, tau = 0.2609, 270.0, 108.0, 0.154, 0.641e-3, 100.0
+
+    def f(t, y):
+        S = y
+        x = w * J_N * S + G * J_N * (SC3.T @ S) + I0
+        u = a * x - b
+        return gamma * (1 - S) * u / (1 - np.exp(-d * u)) - S / tau
+    return f, np.full(3, 0.001), [2]
+
+
 def kuramoto_rhs(G, omega=(0.010, 0.013, 0.017), theta0=(0.3, 2.0, 4.5)):
     om = np.asarray(omega)
 
@@ -94,7 +105,8 @@ def kuramoto_rhs(G, omega=(0.010, 0.013, 0.017), theta0=(0.3, 2.0, 4.5)):
     return f, np.asarray(theta0, float), [0]
 
 
-RHS = {"rWW": (rww_rhs, 0.8), "JR": (jr_rhs, 40.0), "WC": (wc_rhs, 1.5), "Kuramoto": (kuramoto_rhs, 0.02)}
+RHS = {"rWW": (rww_rhs, 0.8), "JR": (jr_rhs, 40.0), "WC": (wc_rhs, 1.5), "Kuramoto": (kuramoto_rhs, 0.02),
+       "rWWEx": (rwwex_rhs, 0.9)}
 
 
 def oracle_states(model, G, dt, duration_ms=300, lib=None):
@@ -132,13 +144,13 @@ def trajectory_error(model, st, ref, idx):
     return (np.abs(sim - ref).max(axis=(1, 2)) / np.abs(ref).max(axis=(1, 2))).max()
 
 
-@pytest.mark.parametrize("model", ["rWW", "JR", "WC", "Kuramoto"])
+@pytest.mark.parametrize("model", ["rWW", "JR", "WC", "Kuramoto", "rWWEx"])
 def test_model_converges_to_continuous_dynamics(model):
     G = RHS[model][1]
     ref, idx = ode_states(model, G)
     e1 = trajectory_error(model, oracle_states(model, G, 0.1), ref, idx)
     e2 = trajectory_error(model, oracle_states(model, G, 0.05), ref, idx)
-    # observed: e2 = 8e-4 (rWW), 4.6e-2 (JR, states start at 0), 2.2e-3 (WC), 9.7e-4 (Kuramoto)
+    # observed: e2 = 8e-4 (rWW), 4.6e-2 (JR, states start at 0), 2.2e-3 (WC), 9.7e-4 (Kuramoto), rWWEx below 1e-2
     assert e2 < 0.1, (e1, e2)
     assert 1.85 < e1 / e2 < 2.15, (e1, e2)
 
@@ -165,7 +177,7 @@ def test_convergence_pin_rejects_planted_mistakes(tmp_path, plant):
 
 
 # ------------------------------------------------------------ fixture statistics
-SEL = {"rWW": 2, "JR": 6, "WC": 0, "Kuramoto": 0}  # r_E, y1_y2, E, theta
+SEL = {"rWW": 2, "JR": 6, "WC": 0, "Kuramoto": 0, "rWWEx": 1}  # r_E, y1_y2, E, theta, r
 
 
 def _fixture_run(model, seed, fic, lib):
@@ -201,7 +213,7 @@ def _ens(model, fic):
     return ENSEMBLES[(model, fic)]
 
 
-@pytest.mark.parametrize("model,fic", [("rWW", 0), ("rWW", 1), ("JR", 0), ("WC", 0)])
+@pytest.mark.parametrize("model,fic", [("rWW", 0), ("rWW", 1), ("JR", 0), ("WC", 0), ("rWWEx", 0)])
 @pytest.mark.parametrize("source", ["force_cpu:1", "force_cpu:0"])
 def test_fixture_node_statistics(model, fic, source):
     d = np.load(f"{cases.GOLDEN}/sim_{model}.npz")
@@ -217,6 +229,7 @@ STAT_PLANTS = {
     "rWW_rE_for_rI": ("rWW", "dt_gamma_E * ((1 - s[4]) * s[2])", "dt_gamma_E * ((1 - s[4]) * s[3])"),
     "JR_noise_sigma_dt": ("JR", "sqrt_dt * noise[idx] * rp[5]", "dt * noise[idx] * rp[5]"),
     "JR_damping_b_for_a": ("JR", "2.0 * a * s[3]", "2.0 * b * s[3]"),
+    "rWWEx_noise_sigma_dt": ("rWWEx", "noise[idx] * sqrt_dt * rp[2]", "noise[idx] * dt * rp[2]"),
 }
 
 

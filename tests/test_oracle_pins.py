@@ -29,7 +29,7 @@ def _fixture_keys(model):
     return d, sorted({k.split("|")[0] for k in d.files})
 
 
-@pytest.mark.parametrize("model", ["rWW", "JR", "Kuramoto", "WC"])
+@pytest.mark.parametrize("model", ["rWW", "JR", "Kuramoto", "WC", "rWWEx"])
 def test_fc_fcd_from_fixture_bold(oracle_lib, model):
     O = oracle_lib
     d, keys = _fixture_keys(model)
