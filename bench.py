@@ -409,8 +409,9 @@ def run_c4(args):
     simulations at dt 0.1 ms, loss (1 - FC corr) + mean |FCD quantile
     differences| (64 quantiles; FCD windows 42 volumes, step 7, 41 volumes of
     burn-in).  CMA-ES (popsize 64, sigma 0.5) runs for --c4-gens generations;
-    Adam on 256 starts (one adjoint pass per step, checkpoint every 1000 BW
-    steps) runs until it reaches CMA-ES's best loss.  value = CMA-ES wall time
+    Adam on 256 starts (one adjoint pass per step, checkpoint every 1,000
+    integration steps = 100 BW steps; loss and its gradient by the loss
+    kernel) runs until it reaches CMA-ES's best loss.  value = CMA-ES wall time
     to its best loss / gradient wall time to the same loss (higher is better)."""
     import torch
     import cases
@@ -420,7 +421,7 @@ def run_c4(args):
     emp = empirical_bold(N, int(dur * 1000) // int(TR_s * 1000), seed=4)
     bounds = {"G": (0.001, 10.0), "w_p": (0.0, 2.0), "J_N": (0.001, 0.5)}
     nvr = int(30 * 1000 / (TR_s * 1000))
-    mk = lambda n: grad.RwwProblem(sc, np.ones((n, N)), 0.01, dur, TR_s, seed=0, checkpoint_bw_steps=1000)
+    mk = lambda n: grad.RwwProblem(sc, np.ones((n, N)), 0.01, dur, TR_s, seed=0, checkpoint_bw_steps=100)
     fit_cma = fit.RegionalFit(mk(64), emp, bounds, 42, 7, nvr)
     log("c4: CMA-ES")
     h_cma = fit.cmaes_fit(fit_cma, args.c4_gens, seed=1, log=log)

@@ -33,7 +33,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["session.cpp", "integrate_general.hip", "integrate_batched.hip", "fc_fcd.hip", "fc_gram.hip",
-               "gof.hip", "adjoint.hip"]
+               "gof.hip", "adjoint.hip", "loss.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are disabled for all device code

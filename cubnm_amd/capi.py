@@ -57,6 +57,9 @@ def lib():
     L.cubnm_rww_grad_sizes.argtypes = [I, I, I, I, D, D, I, LL, LL, LL]
     L.cubnm_rww_grad_forward.argtypes = [I, I, I, I, D, D, I, I] + [P] * 11
     L.cubnm_rww_grad_backward.argtypes = [I, I, I, I, D, D, I, I] + [P] * 15
+    L.cubnm_fc_fcd_loss_scratch.argtypes = [I, I, I, I, I]
+    L.cubnm_fc_fcd_loss_scratch.restype = ctypes.c_longlong
+    L.cubnm_fc_fcd_loss.argtypes = [P, I, I, I, I, P, P, I, I, P, P, P, I, P, P, P, P]
     L.cubnm_init()
     _LIB = L
     return L
@@ -140,3 +143,16 @@ def rww_grad_backward(n_sims, nodes, time_steps, BOLD_TR, dt, bw_dt, ck, noise_t
     grad_G, grad_w_p, grad_J_N (device addresses)."""
     _check(lib().cubnm_rww_grad_backward(n_sims, nodes, time_steps, BOLD_TR, float(dt), float(bw_dt), ck,
                                          noise_time_steps, *ptrs))
+
+
+def fc_fcd_loss_scratch(bold_len, nodes, n_vols_remove, n_windows, max_window_len):
+    """Scratch doubles per simulation of fc_fcd_loss."""
+    return lib().cubnm_fc_fcd_loss_scratch(bold_len, nodes, n_vols_remove, n_windows, max_window_len)
+
+
+def fc_fcd_loss(bold, n_sims, bold_len, nodes, n_vols_remove, win_start, win_end, n_windows, max_window_len,
+                emp_fc, emp_q, qs, n_quantiles, loss, dbold, scratch, stream=None):
+    """Regional-fit loss (and, with dbold not None, its gradient) of device
+    BOLD; all array arguments are device addresses."""
+    _check(lib().cubnm_fc_fcd_loss(bold, n_sims, bold_len, nodes, n_vols_remove, win_start, win_end, n_windows,
+                                   max_window_len, emp_fc, emp_q, qs, n_quantiles, loss, dbold, scratch, stream))

@@ -213,6 +213,13 @@ PyObject* py_run_simulations(PyObject*, PyObject* args) {
     return result_dict(c);
 }
 
+PyObject* py_run_groups_info(PyObject*, PyObject*) {
+    double t[6];
+    cubnm_run_groups_info(t);
+    return Py_BuildValue("{s:i,s:i,s:i,s:i,s:d,s:i}", "graph", (int)t[0], "ksplit", (int)t[1], "pair", (int)t[2],
+                         "tuned", (int)t[3], "trial_ms", t[4], "shapes_timed", (int)t[5]);
+}
+
 // run_groups(requests, batched_tpw=None) -> list of dicts: each request is a
 // run_simulations argument tuple; all run in one call (cubnm_run_groups).
 PyObject* py_run_groups(PyObject*, PyObject* args) {
@@ -350,6 +357,9 @@ PyMethodDef methods[] = {
      "                model_config, ext_out, states_ts, noise_out, do_delay, force_reinit, use_cpu,\n"
      "                N_SIMS, nodes, time_steps, BOLD_TR, states_sampling, sim_seed, dt, bw_dt) -> dict\n\n"
      "Run a group of simulations on the GPU; see include/cubnm_amd.h for layouts."},
+    {"run_groups_info", py_run_groups_info, METH_NOARGS,
+     "run_groups_info() -> dict: graph (0 none, 1 captured, 2 replayed), ksplit, pair, tuned (0 rule, 1 trials,\n"
+     "2 cached), trial_ms, shapes_timed of the last run_groups call's per-simulation launch"},
     {"run_groups", py_run_groups, METH_VARARGS,
      "run_groups(requests, batched_tpw=None) -> list of dicts\n\n"
      "Run several run_simulations requests (each its own model, node count, duration, config, SCs\n"

@@ -18,8 +18,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <exception>
+#include <fstream>
 #include <map>
+#include <set>
+#include <sys/stat.h>
 #include <memory>
 #include <tuple>
 #include <numeric>
@@ -849,6 +853,86 @@ BWConsts current_bw() {
 
 using namespace cubnm;
 
+// ---------------------------------------------------------------------------
+// Launch shape of the merged per-simulation launch: the autotuner and the
+// captured graph.
+//
+// Autotuner: for a large merged launch (or always, with the "autotune" knob
+// at 1) it times short trials of every candidate shape - k partitions of
+// the coupling sum in {1, 2, 4} (as fit 1024 lanes) x pair mode {as the rule
+// allows, off} - on the real launch with every request cut to a few BW
+// steps, and keeps the fastest.  Choices are cached per (models, largest
+// node count, simulations rounded to a power of two, SC-sharing) in memory
+// and in a text file ($CUBNM_AMD_TUNE_CACHE, else ~/.cache/cubnm_amd/tune.txt)
+// so later processes on the same machine reuse them.  Requests with
+// numerical-FIC restarts are never cut into trials (their restarts write wIE
+// back), so launches holding them keep the rule-based shape.
+//
+// Graph: the launch tables' uploads and the merged launch are captured into
+// a hipGraph the first time a launch signature (every request's arguments,
+// tables, block shape and LDS size) is seen and replayed while the next
+// call's signature is identical (for example the generations of an
+// optimiser, whose parameters change but whose buffers do not).  The tables
+// live in pinned host memory owned by the cached graph.
+struct TuneChoice { int ks = 0; int pair = 0; double trial_ms = 0; };
+std::map<std::string, TuneChoice> g_tune_cache;
+bool g_tune_loaded = false;
+int g_autotune = 0;   // -1 never, 0 large launches, 1 always
+int g_group_graph = 1;  // capture and replay the merged launch
+
+std::string tune_cache_path() {
+    if (const char* p = std::getenv("CUBNM_AMD_TUNE_CACHE")) return p;
+    const char* h = std::getenv("HOME");
+    return std::string(h ? h : "/tmp") + "/.cache/cubnm_amd/tune.txt";
+}
+
+void tune_cache_load() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    std::ifstream f(tune_cache_path());
+    std::string key;
+    TuneChoice c;
+    while (f >> key >> c.ks >> c.pair >> c.trial_ms) g_tune_cache[key] = c;
+}
+
+void tune_cache_store(const std::string& key, const TuneChoice& c) {
+    g_tune_cache[key] = c;
+    try {
+        const std::string path = tune_cache_path();
+        const size_t slash = path.rfind('/');
+        if (slash != std::string::npos) {
+            std::string dir = path.substr(0, slash), cur;
+            for (size_t i = 1; i <= dir.size(); i++)
+                if (i == dir.size() || dir[i] == '/') { cur = dir.substr(0, i); mkdir(cur.c_str(), 0755); }
+        }
+        std::ofstream f(path, std::ios::app);
+        f << key << " " << c.ks << " " << c.pair << " " << c.trial_ms << "\n";
+    } catch (...) {
+    }
+}
+
+struct GroupGraph {
+    std::vector<char> sig;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    void* pinned[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::unique_ptr<DevBuf[]> bufs;
+    int replays = 0;
+    void release() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void*& p : pinned) if (p) { (void)hipHostFree(p); p = nullptr; }
+        exec = nullptr; graph = nullptr;
+        bufs.reset();
+        sig.clear();
+    }
+};
+GroupGraph g_group_graph_cache;
+// last call: [0] graph (0 none, 1 captured, 2 replayed), [1] k partitions,
+// [2] pair mode (1 allowed by the rule, 0 off), [3] tuned (0 rule, 1 trials
+// run, 2 cached choice), [4] trial time ms, [5] candidates timed
+double g_group_info[6] = {0, 0, 0, 0, 0, 0};
+
 extern "C" {
 
 const char* cubnm_last_error(void) { return g_err.c_str(); }
@@ -884,6 +968,16 @@ int cubnm_set_tuning(const char* key, int value) {
     if (std::strcmp(key, "general_pair") == 0) {
         if (value < -1 || value > 1) { g_err = "general_pair must be -1 (never), 0 (auto) or 1 (whenever eligible)"; return -1; }
         g_general_pair = value;
+        return 0;
+    }
+    if (std::strcmp(key, "autotune") == 0) {
+        if (value < -1 || value > 1) { g_err = "autotune must be -1 (never), 0 (large launches) or 1 (always)"; return -1; }
+        g_autotune = value;
+        return 0;
+    }
+    if (std::strcmp(key, "group_graph") == 0) {
+        if (value < 0 || value > 1) { g_err = "group_graph must be 0 (off) or 1 (capture and replay)"; return -1; }
+        g_group_graph = value;
         return 0;
     }
     if (std::strcmp(key, "fc_mode") == 0) {
@@ -1023,6 +1117,7 @@ int cubnm_prepare_groups(int n_groups, cubnm_group* groups) {
     }
 }
 
+
 int cubnm_run_groups(int n_groups, cubnm_group* groups) {
     int staged = 0;
     std::vector<int> order(std::max(0, n_groups));
@@ -1090,15 +1185,14 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
             b.n_vols += (long long)s.bold_len * s.N_SIMS;
             b.members.push_back(i);
         }
-        for (auto& kv : buckets) {
-            Bucket& b = kv.second;
-            int nmax = 1, sims = 0;
-            for (int i : b.members) {
-                nmax = std::max(nmax, g_pool[i]->nodes);
-                sims += g_pool[i]->N_SIMS;
-            }
-            b.ks = general_ksplit(nmax, g_general_ksplit);
+        // shape of a bucket's launch for k partitions ks and pair policy
+        // (1: the rule's latency-bound condition, 0: off)
+        auto shape_bucket = [&](Bucket& b, int ks, int pair_policy) {
+            int sims = 0;
+            for (int i : b.members) sims += g_pool[i]->N_SIMS;
+            b.ks = ks;
             b.threads = b.ks * b.np;
+            b.lds = 0;
             for (int i : b.members) {
                 Session& s = *g_pool[i];
                 // small SCs staged; larger ones read through the caches, in
@@ -1106,10 +1200,18 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
                 const size_t with_sc = general_lds_bytes(s.model->id, s.nodes, b.np, b.ks, true, false, b.threads);
                 s.iargs.sc_lds = with_sc <= 64 * 1024;
                 s.iargs.ksplit = b.ks;
-                s.iargs.pair = general_pair_ok(s.iargs, b.threads, sims <= 512 || s.N_SIMS >= 4 * s.N_SCs);
+                s.iargs.pair = pair_policy && general_pair_ok(s.iargs, b.threads,
+                                                              sims <= 512 || s.N_SIMS >= 4 * s.N_SCs);
                 b.lds = std::max(b.lds, general_lds_bytes(s.model->id, s.nodes, b.np, b.ks, s.iargs.sc_lds,
                                                           s.iargs.pair != 0, b.threads));
             }
+        };
+        for (int q = 0; q < 6; q++) g_group_info[q] = 0;
+        for (auto& kv : buckets) {
+            Bucket& b = kv.second;
+            int nmax = 1;
+            for (int i : b.members) nmax = std::max(nmax, g_pool[i]->nodes);
+            shape_bucket(b, general_ksplit(nmax, g_general_ksplit), 1);
         }
         // host copies of the launch tables stay alive until the next call
         static std::vector<std::vector<char>> host_tables;
@@ -1141,30 +1243,184 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups) {
                 const int2 v = make_int2(blk[q].second.first, blk[q].second.second);
                 std::memcpy(blocks.data() + q * sizeof(int2), &v, sizeof(int2));
             }
-            std::unique_ptr<DevBuf[]> bufs(new DevBuf[4]);
-            bufs[0].ensure(args.size());
-            bufs[1].ensure(consts.size());
-            bufs[2].ensure(blocks.size());
-            bufs[3].ensure(models.size());
             b.stream = g_pool[b.members[0]]->stream;
-            check(hipMemcpyAsync(bufs[0].p, args.data(), args.size(), hipMemcpyHostToDevice, b.stream), "args H2D");
-            check(hipMemcpyAsync(bufs[1].p, consts.data(), consts.size(), hipMemcpyHostToDevice, b.stream), "consts H2D");
-            check(hipMemcpyAsync(bufs[2].p, blocks.data(), blocks.size(), hipMemcpyHostToDevice, b.stream), "blocks H2D");
-            check(hipMemcpyAsync(bufs[3].p, models.data(), models.size(), hipMemcpyHostToDevice, b.stream), "models H2D");
+            // argument table of the bucket's members as shaped now, with every
+            // request cut to `trial_bw` BW steps when trial_bw > 0
+            auto args_table = [&](int trial_bw) {
+                std::vector<char> t;
+                for (int i : b.members) {
+                    IntegrateArgs a = g_pool[i]->iargs;
+                    if (trial_bw > 0) a.bw_it = std::min(a.bw_it, trial_bw);
+                    const char* ap = reinterpret_cast<const char*>(&a);
+                    t.insert(t.end(), ap, ap + sizeof(IntegrateArgs));
+                }
+                return t;
+            };
+            // ---- autotuner
+            {
+                long long work = 0;
+                bool restarts = false;
+                std::set<int> mids;
+                int nmax = 1, sims = 0;
+                bool shared = true;
+                for (int i : b.members) {
+                    const Session& s = *g_pool[i];
+                    work += (long long)s.N_SIMS * s.bw_it * s.inner_it * s.nodes;
+                    restarts = restarts || (s.model->id == MODEL_RWW && s.rww.do_fic && s.rww.max_fic_trials > 0);
+                    mids.insert(s.model->id);
+                    nmax = std::max(nmax, s.nodes);
+                    sims += s.N_SIMS;
+                    shared = shared && s.N_SIMS >= 4 * s.N_SCs;
+                }
+                const bool want = !restarts && g_general_ksplit == 0 && g_general_pair == 0 &&
+                                  (g_autotune > 0 || (g_autotune == 0 && work >= 20000000000LL));
+                if (want) {
+                    tune_cache_load();
+                    int sims_p2 = 1;
+                    while (sims_p2 < sims) sims_p2 <<= 1;
+                    std::string key = "general";
+                    for (int m : mids) key += "," + std::to_string(m);
+                    key += "|N" + std::to_string(nmax) + "|S" + std::to_string(sims_p2) + (shared ? "|shared" : "|own");
+                    auto it = g_tune_cache.find(key);
+                    TuneChoice best;
+                    if (it != g_tune_cache.end() && it->second.ks * b.np <= 1024) {
+                        best = it->second;
+                        g_group_info[3] = 2;
+                        g_group_info[4] = best.trial_ms;
+                    } else {
+                        // trials: every request cut to 20 BW steps (200 steps at the usual dt)
+                        std::unique_ptr<DevBuf[]> tb(new DevBuf[4]);
+                        tb[1].ensure(consts.size()); tb[2].ensure(blocks.size()); tb[3].ensure(models.size());
+                        check(hipMemcpyAsync(tb[1].p, consts.data(), consts.size(), hipMemcpyHostToDevice, b.stream), "H2D");
+                        check(hipMemcpyAsync(tb[2].p, blocks.data(), blocks.size(), hipMemcpyHostToDevice, b.stream), "H2D");
+                        check(hipMemcpyAsync(tb[3].p, models.data(), models.size(), hipMemcpyHostToDevice, b.stream), "H2D");
+                        hipEvent_t e0, e1;
+                        check(hipEventCreate(&e0), "event");
+                        check(hipEventCreate(&e1), "event");
+                        best.trial_ms = 1e300;
+                        int timed = 0;
+                        for (int ks : {1, 2, 4}) {
+                            if (ks * b.np > 1024) continue;
+                            for (int pr : {1, 0}) {
+                                shape_bucket(b, ks, pr);
+                                std::vector<char> targs = args_table(20);
+                                tb[0].ensure(targs.size());
+                                check(hipMemcpyAsync(tb[0].p, targs.data(), targs.size(), hipMemcpyHostToDevice, b.stream),
+                                      "H2D");
+                                check(hipEventRecord(e0, b.stream), "event");
+                                check(launch_integrate_general_multi(tb[0].as<IntegrateArgs>(), tb[1].p, tb[3].as<int>(),
+                                                                     tb[2].as<int2>(), (int)blk.size(), b.threads, b.lds,
+                                                                     nullptr, b.stream),
+                                      "autotune trial launch");
+                                check(hipEventRecord(e1, b.stream), "event");
+                                check(hipEventSynchronize(e1), "autotune trial");
+                                const double ms = ev_ms(e0, e1);
+                                timed++;
+                                if (ms < best.trial_ms) { best.trial_ms = ms; best.ks = ks; best.pair = pr; }
+                                bool any_pair = false;
+                                for (int i : b.members) any_pair = any_pair || g_pool[i]->iargs.pair;
+                                if (!any_pair) break;  // pair mode changes nothing for this launch
+                            }
+                        }
+                        hipEventDestroy(e0); hipEventDestroy(e1);
+                        tune_cache_store(key, best);
+                        g_group_info[3] = 1;
+                        g_group_info[4] = best.trial_ms;
+                        g_group_info[5] = timed;
+                        if (verbose)
+                            std::fprintf(stderr, "cubnm_run_groups: autotuner %s: %d k partitions, pair mode %s "
+                                         "(%d shapes timed, best trial %.2f ms)\n", key.c_str(), best.ks,
+                                         best.pair ? "allowed" : "off", timed, best.trial_ms);
+                    }
+                    shape_bucket(b, best.ks, best.pair);
+                }
+                g_group_info[1] = b.ks;
+                g_group_info[2] = 0;
+                for (int i : b.members) g_group_info[2] = g_group_info[2] || g_pool[i]->iargs.pair;
+            }
+            args = args_table(0);
+            for (int q = 0; q < 8; q++) progress_host[q] = 0;
+            // ---- graph: capture the table uploads and the launch, or replay
+            std::vector<char> sig;
+            auto put = [&](const void* p, size_t n) {
+                const char* c = static_cast<const char*>(p);
+                sig.insert(sig.end(), c, c + n);
+            };
+            put(args.data(), args.size()); put(consts.data(), consts.size());
+            put(blocks.data(), blocks.size()); put(models.data(), models.size());
+            put(&b.threads, sizeof(b.threads)); put(&b.lds, sizeof(b.lds)); put(&b.stream, sizeof(b.stream));
+            GroupGraph& gg = g_group_graph_cache;
+            bool replay = g_group_graph && buckets.size() == 1 && gg.exec && gg.sig == sig;
+            std::unique_ptr<DevBuf[]> bufs;
             check(hipEventCreate(&b.start), "event");
             check(hipEventCreate(&b.stop), "event");
-            check(hipEventRecord(b.start, b.stream), "event");
             b.n_sims = (int)blk.size();
-            check(launch_integrate_general_multi(bufs[0].as<IntegrateArgs>(), bufs[1].p, bufs[3].as<int>(),
-                                                 bufs[2].as<int2>(), (int)blk.size(), b.threads, b.lds, progress_dev,
-                                                 b.stream),
-                  "integrate_general_multi launch");
-            check(hipEventRecord(b.stop, b.stream), "event");
+            if (replay) {
+                check(hipEventRecord(b.start, b.stream), "event");
+                check(hipGraphLaunch(gg.exec, b.stream), "hipGraphLaunch");
+                check(hipEventRecord(b.stop, b.stream), "event");
+                gg.replays++;
+                g_group_info[0] = 2;
+            } else {
+                if (g_group_graph) gg.release();  // a new capture replaces the cached graph
+                bufs.reset(new DevBuf[4]);
+                bufs[0].ensure(args.size());
+                bufs[1].ensure(consts.size());
+                bufs[2].ensure(blocks.size());
+                bufs[3].ensure(models.size());
+                bool captured = false;
+                if (g_group_graph && buckets.size() == 1) {
+                    // pinned copies of the tables: the captured copies read them on every replay
+                    const std::vector<char>* src[4] = {&args, &consts, &blocks, &models};
+                    bool ok = true;
+                    for (int q = 0; q < 4 && ok; q++) {
+                        ok = hipHostMalloc(&gg.pinned[q], std::max<size_t>(1, src[q]->size()), hipHostMallocDefault) ==
+                             hipSuccess;
+                        if (ok) std::memcpy(gg.pinned[q], src[q]->data(), src[q]->size());
+                    }
+                    if (ok && hipStreamBeginCapture(b.stream, hipStreamCaptureModeRelaxed) == hipSuccess) {
+                        for (int q = 0; q < 4; q++)
+                            (void)hipMemcpyAsync(bufs[q].p, gg.pinned[q], src[q]->size(), hipMemcpyHostToDevice,
+                                                 b.stream);
+                        const hipError_t le = launch_integrate_general_multi(
+                            bufs[0].as<IntegrateArgs>(), bufs[1].p, bufs[3].as<int>(), bufs[2].as<int2>(),
+                            (int)blk.size(), b.threads, b.lds, progress_dev, b.stream);
+                        hipGraph_t graph = nullptr;
+                        const hipError_t ee = hipStreamEndCapture(b.stream, &graph);
+                        if (le == hipSuccess && ee == hipSuccess && graph &&
+                            hipGraphInstantiate(&gg.exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                            gg.graph = graph;
+                            captured = true;
+                        } else if (graph) {
+                            (void)hipGraphDestroy(graph);
+                        }
+                        (void)hipGetLastError();
+                    }
+                }
+                check(hipEventRecord(b.start, b.stream), "event");
+                if (captured) {
+                    check(hipGraphLaunch(gg.exec, b.stream), "hipGraphLaunch");
+                    gg.sig = sig;
+                    gg.bufs = std::move(bufs);
+                    g_group_info[0] = 1;
+                } else {
+                    if (g_group_graph) gg.release();
+                    check(hipMemcpyAsync(bufs[0].p, args.data(), args.size(), hipMemcpyHostToDevice, b.stream), "args H2D");
+                    check(hipMemcpyAsync(bufs[1].p, consts.data(), consts.size(), hipMemcpyHostToDevice, b.stream), "consts H2D");
+                    check(hipMemcpyAsync(bufs[2].p, blocks.data(), blocks.size(), hipMemcpyHostToDevice, b.stream), "blocks H2D");
+                    check(hipMemcpyAsync(bufs[3].p, models.data(), models.size(), hipMemcpyHostToDevice, b.stream), "models H2D");
+                    check(launch_integrate_general_multi(bufs[0].as<IntegrateArgs>(), bufs[1].p, bufs[3].as<int>(),
+                                                         bufs[2].as<int2>(), (int)blk.size(), b.threads, b.lds,
+                                                         progress_dev, b.stream),
+                          "integrate_general_multi launch");
+                }
+                check(hipEventRecord(b.stop, b.stream), "event");
+            }
             host_tables.push_back(std::move(args));
             host_tables.push_back(std::move(consts));
             host_tables.push_back(std::move(blocks));
             host_tables.push_back(std::move(models));
-            multi_bufs.push_back(std::move(bufs));
+            if (bufs) multi_bufs.push_back(std::move(bufs));
             if (verbose)
                 std::fprintf(stderr, "cubnm_run_groups: %zu requests, %zu simulations in one per-simulation launch "
                              "(%d threads in %d k partitions, %zu B LDS)\n",
@@ -1307,3 +1563,8 @@ int cubnm_run_device(const double* d_SC, const int* d_SCi, const double* d_SC_di
 }
 
 }  // extern "C"
+
+extern "C" int cubnm_run_groups_info(double* out6) {
+    for (int i = 0; i < 6; i++) out6[i] = g_group_info[i];
+    return 0;
+}

@@ -2,8 +2,11 @@
 kernel versus CMA-ES, on the same forward model, noise and loss.
 
 * Loss (per parameter set): (1 - Pearson(FC_sim, FC_emp)) plus the mean
-  absolute difference of 64 quantiles of the FCD lower triangles, all in
-  torch on the simulated BOLD (``cubnm_amd.grad.rww_bold``).
+  absolute difference of 64 quantiles of the FCD lower triangles of the
+  simulated BOLD (``cubnm_amd.grad.rww_bold``).  ``FcFcdLoss`` evaluates it
+  and its gradient with respect to BOLD in one hand-written kernel
+  (cubnm_fc_fcd_loss); ``fit_loss`` is the same loss in torch operations,
+  kept as the reference the kernel is tested against.
 * Parameters live in the unit cube, mapped linearly to their bounds, as the
   reference's optimisers do (optimize.py:432-537 maps [0, 1] to each
   parameter's range).
@@ -49,6 +52,66 @@ def fit_loss(bold, emp_fc, emp_fcd_q, windows, n_vols_remove, n_quantiles=64):
     qs = torch.linspace(0, 1, n_quantiles, dtype=bold.dtype, device=bold.device)
     q = torch.quantile(fcd_tril(bold, windows), qs, dim=1).T
     return l_fc + (q - emp_fcd_q[None, :]).abs().mean(dim=1)
+
+
+class FcFcdLoss:
+    """The loss of ``fit_loss`` on device BOLD (n_sims, bold_len, nodes),
+    evaluated by the library's kernel; ``value(bold)`` gives the loss per
+    simulation, ``value_and_grad(bold)`` also dL/dBOLD."""
+
+    def __init__(self, emp_fc, emp_q, windows, n_vols_remove, n_sims, bold_len, nodes, n_quantiles=64):
+        from . import capi
+        self.capi = capi
+        dev = emp_fc.device
+        if any(s < 0 or e >= bold_len or e <= s for s, e in windows) or len(windows) < 2:
+            raise ValueError("windows must lie within the BOLD volumes, at least two, each two volumes or more")
+        self.S, self.T, self.N, self.nvr = n_sims, bold_len, nodes, n_vols_remove
+        self.W = len(windows)
+        self.Lmax = max(e - s + 1 for s, e in windows)
+        self.ws = torch.tensor([w[0] for w in windows], dtype=torch.int32, device=dev)
+        self.we = torch.tensor([w[1] for w in windows], dtype=torch.int32, device=dev)
+        self.emp_fc = emp_fc.contiguous()
+        self.emp_q = emp_q.contiguous()
+        self.qs = torch.linspace(0, 1, n_quantiles, dtype=torch.float64, device=dev)
+        self.nq = n_quantiles
+        per = capi.fc_fcd_loss_scratch(bold_len, nodes, n_vols_remove, self.W, self.Lmax)
+        self.scratch = torch.empty(n_sims * per, dtype=torch.float64, device=dev)
+
+    def _run(self, bold, with_grad):
+        bold = bold.contiguous()
+        assert bold.shape == (self.S, self.T, self.N) and bold.dtype == torch.float64
+        loss = torch.empty(self.S, dtype=torch.float64, device=bold.device)
+        dbold = torch.empty_like(bold) if with_grad else None
+        torch.cuda.synchronize()
+        self.capi.fc_fcd_loss(bold.data_ptr(), self.S, self.T, self.N, self.nvr, self.ws.data_ptr(),
+                              self.we.data_ptr(), self.W, self.Lmax, self.emp_fc.data_ptr(), self.emp_q.data_ptr(),
+                              self.qs.data_ptr(), self.nq, loss.data_ptr(),
+                              dbold.data_ptr() if with_grad else None, self.scratch.data_ptr())
+        torch.cuda.synchronize()
+        return loss, dbold
+
+    def value(self, bold):
+        return self._run(bold, False)[0]
+
+    def value_and_grad(self, bold):
+        return self._run(bold, True)
+
+    def __call__(self, bold):
+        """Differentiable loss: backward multiplies the kernel's dL/dBOLD."""
+        return _FcFcdLossFn.apply(bold, self)
+
+
+class _FcFcdLossFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, bold, lossobj):
+        loss, dbold = lossobj.value_and_grad(bold.detach())
+        ctx.save_for_backward(dbold)
+        return loss
+
+    @staticmethod
+    def backward(ctx, gl):
+        (dbold,) = ctx.saved_tensors
+        return dbold * gl[:, None, None], None
 
 
 def windows_for(bold_len, window, step, n_vols_remove):
@@ -129,6 +192,8 @@ class RegionalFit:
         self.emp_fc = grad.fc_tril(eb, n_vols_remove)[0]
         qs = torch.linspace(0, 1, 64, dtype=torch.float64, device=dev)
         self.emp_q = torch.quantile(fcd_tril(eb, self.windows)[0], qs)
+        self.loss = FcFcdLoss(self.emp_fc, self.emp_q, self.windows, n_vols_remove, problem.n_sims,
+                              problem.bold_len, self.N)
 
     def _split(self, theta):
         return theta[:, 0].contiguous(), theta[:, 1:1 + self.N].contiguous(), theta[:, 1 + self.N:].contiguous()
@@ -137,13 +202,13 @@ class RegionalFit:
         with torch.no_grad():
             theta = self.lo + (self.hi - self.lo) * U
             bold = self.p.forward(*self._split(theta))
-            return fit_loss(bold, self.emp_fc, self.emp_q, self.windows, self.nvr)
+            return self.loss.value(bold)
 
     def loss_and_grad(self, U):
         Uv = U.clone().requires_grad_(True)
         theta = self.lo + (self.hi - self.lo) * Uv
         G, wp, JN = self._split(theta)
-        loss = fit_loss(grad.rww_bold(self.p, G, wp, JN), self.emp_fc, self.emp_q, self.windows, self.nvr)
+        loss = self.loss(grad.rww_bold(self.p, G, wp, JN))
         loss.sum().backward()
         return loss.detach(), Uv.grad.detach()
 

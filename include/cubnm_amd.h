@@ -134,6 +134,12 @@ int cubnm_run_groups(int n_groups, cubnm_group* groups);
  * cubnm_last_timings; [2] is the integration kernel time of the launch that
  * ran the group (the one merged launch for per-simulation groups). */
 int cubnm_group_timings(int group, double* out6);
+/* How the last cubnm_run_groups call ran its merged per-simulation launch
+ * (no counterpart in the reference): out6 = [graph: 0 not captured, 1
+ * captured and launched, 2 replayed; k partitions of the coupling sum; pair
+ * mode (1 used by some request); launch shape from: 0 the rule, 1 autotuner
+ * trials, 2 the autotuner's cached choice; best trial ms; shapes timed]. */
+int cubnm_run_groups_info(double* out6);
 
 /* Timings of the last run (milliseconds): [0] parameter prep on host,
  * [1] host->device copies, [2] integration kernel, [3] FC/FCD kernel,
@@ -157,6 +163,14 @@ int cubnm_last_integrator(void);
  *                   latency: SCs shared or few simulations), 1 = whenever
  *                   eligible (even N >= 96, SC in global memory, no delay),
  *                   -1 = never;
+ *   "autotune"    = merged per-simulation launch of cubnm_run_groups: time
+ *                   short trials of its k partitions and pair mode and keep
+ *                   the fastest, cached per launch kind in memory and in
+ *                   $CUBNM_AMD_TUNE_CACHE (else ~/.cache/cubnm_amd/tune.txt):
+ *                   -1 never, 0 = launches of >= 2e10 node-steps, 1 always;
+ *   "group_graph" = capture that launch and its table uploads into a
+ *                   hipGraph and replay it while the launch is unchanged
+ *                   (1, default) or launch directly (0);
  *   "fc_mode"     = FC/FCD path: 0 = auto (per-simulation LDS kernel when a
  *                   simulation's BOLD fits in LDS, else FP64 Gram products),
  *                   1 = FP64 Gram products on MFMA, 2 = split-BF16 Gram
@@ -195,6 +209,23 @@ int cubnm_rww_grad_backward(int n_sims, int nodes, int time_steps, int BOLD_TR, 
                             const double* d_JN, const double* d_wIE, const double* d_sigma, const double* d_noise,
                             const int* d_sh_nodes, const int* d_sh_ts, const double* d_ckpt, const double* d_dbold,
                             double* d_scratch, double* d_gG, double* d_gwp, double* d_gJN);
+
+/* The regional-fit loss L = (1 - r(FC_sim, FC_emp)) + mean_q |Q_q(FCD_sim) -
+ * E_q| of each simulation's BOLD and, when d_dbold is not NULL, its gradient
+ * dL/dBOLD by a hand-written adjoint (FC over the volumes after
+ * n_vols_remove; FCD between the FCs of windows [d_win_start[k],
+ * d_win_end[k]] (inclusive, within [0, bold_len)); Q_q the linear-interpolation
+ * quantile at d_qs[q] (n - 1) of the sorted FCD values).  No counterpart in
+ * the reference (the differentiable objective of gradient fitting, checked
+ * against torch autograd of the same loss).  Device pointers; d_scratch holds
+ * cubnm_fc_fcd_loss_scratch(...) doubles per simulation; queued on `stream`
+ * (a hipStream_t, NULL = null stream) without waiting.  A NaN FC or FCD gives
+ * a NaN loss and a zero gradient. */
+long long cubnm_fc_fcd_loss_scratch(int bold_len, int nodes, int n_vols_remove, int n_windows, int max_window_len);
+int cubnm_fc_fcd_loss(const double* d_bold, int n_sims, int bold_len, int nodes, int n_vols_remove,
+                      const int* d_win_start, const int* d_win_end, int n_windows, int max_window_len,
+                      const double* d_emp_fc, const double* d_emp_q, const double* d_qs, int n_quantiles,
+                      double* d_loss, double* d_dbold, double* d_scratch, void* stream);
 
 /* Goodness of fit on the GPU (utils.py:306-507 fc_corr_device and
  * fcd_ks_device, sim/base.py:1084-1207): Pearson r between each simulated FC

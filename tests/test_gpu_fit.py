@@ -44,3 +44,52 @@ def test_cmaes_and_adam_run():
     assert len(h) == 4 and all(b2 <= b1 for (_, b1), (_, b2) in zip(h, h[1:]))
     h2 = fit.adam_fit(f, 3, seed=0)
     assert len(h2) == 3 and np.isfinite(h2[-1][1])
+
+
+def _loss_case(S=3, T=40, N=7, nvr=6, seed=0):
+    import torch
+    from cubnm_amd import fit, grad
+    rng = np.random.default_rng(seed)
+    bold = torch.tensor(rng.standard_normal((S, T, N)).cumsum(axis=1) * 0.1 + rng.standard_normal((S, 1, N)),
+                        dtype=torch.float64, device="cuda")
+    windows = fit.windows_for(T, 6, 3, nvr)
+    emp_bold = torch.tensor(rng.standard_normal((1, T, N)).cumsum(axis=1), dtype=torch.float64, device="cuda")
+    emp_fc = grad.fc_tril(emp_bold, nvr)[0]
+    qs = torch.linspace(0, 1, 64, dtype=torch.float64, device="cuda")
+    emp_q = torch.quantile(fit.fcd_tril(emp_bold, windows)[0], qs)
+    return bold, windows, emp_fc, emp_q, nvr
+
+
+def test_loss_kernel_matches_torch_autograd():
+    """The hand-written loss and its adjoint (cubnm_fc_fcd_loss) against the
+    same loss written in torch operations (fit.fit_loss) and its autograd,
+    float64 on the CPU: loss 1e-12, gradient 1e-10 of its largest entry."""
+    import torch
+    from cubnm_amd import fit
+    bold, windows, emp_fc, emp_q, nvr = _loss_case()
+    S, T, N = bold.shape
+    lossobj = fit.FcFcdLoss(emp_fc, emp_q, windows, nvr, S, T, N)
+    loss, dbold = lossobj.value_and_grad(bold)
+    b = bold.cpu().clone().requires_grad_(True)
+    ref = fit.fit_loss(b, emp_fc.cpu(), emp_q.cpu(), windows, nvr)
+    ref.sum().backward()
+    np.testing.assert_allclose(loss.cpu().numpy(), ref.detach().numpy(), rtol=0, atol=1e-12)
+    scale = float(b.grad.abs().max())
+    assert float((dbold.cpu() - b.grad).abs().max()) <= 1e-10 * scale
+    np.testing.assert_array_equal(lossobj.value(bold).cpu().numpy(), loss.cpu().numpy())
+    # through autograd: the backward multiplies by the incoming gradient
+    bv = bold.clone().requires_grad_(True)
+    w = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64, device="cuda")
+    (lossobj(bv) * w).sum().backward()
+    np.testing.assert_allclose(bv.grad.cpu().numpy(), (dbold * w[:, None, None]).cpu().numpy(), rtol=0, atol=0)
+
+
+def test_loss_kernel_nan_simulation():
+    from cubnm_amd import fit
+    bold, windows, emp_fc, emp_q, nvr = _loss_case(seed=1)
+    bold[1, :, 2] = 0.25  # a constant node: undefined correlations
+    S, T, N = bold.shape
+    loss, dbold = fit.FcFcdLoss(emp_fc, emp_q, windows, nvr, S, T, N).value_and_grad(bold)
+    loss = loss.cpu().numpy()
+    assert np.isnan(loss[1]) and np.isfinite(loss[[0, 2]]).all()
+    assert float(dbold[1].abs().max()) == 0.0 and float(dbold[0].abs().max()) > 0.0

@@ -131,3 +131,66 @@ def test_c3_shaped_large_networks_match_oracle(pair):
         s, rs = g["_sim_states"], ref["_sim_states"]
         scale = np.maximum(np.nanmax(np.abs(rs), axis=(1, 2), keepdims=True), 1e-300)
         assert cases.max_abs_err(s / scale, rs / scale) <= tol, msg
+
+
+def _per_sim_requests(g_shift):
+    # requests that all run on the per-simulation integrator (own SCs, or a
+    # delay), so the merged launch is the whole integration
+    reqs = []
+    for model, N, S, seed in [("rWW", 12, 3, 31), ("JR", 9, 2, 32), ("Kuramoto", 7, 2, 33)]:
+        scs = np.stack([cases.synthetic_sc(N, seed=seed * 10 + i) for i in range(S)])
+        gp, rp = cases.params(model, S, N, G=np.linspace(0.3, 0.9, S) + g_shift, do_fic=False)
+        cfg = cases.model_config(model, window_size=4, window_step=2, bold_remove_s=2, do_fic=False)
+        reqs.append(cases.sim_args(model, scs, gp, rp, cfg, S, duration_s=8, sc_indices=np.arange(S),
+                                   sim_seed=seed % 3))
+    return reqs
+
+
+def test_group_graph_replays_match_direct_launches():
+    """The merged launch is captured into a hipGraph on the first call and
+    replayed on later calls with the same shapes (new parameter values each
+    time); every replay equals a direct launch of the same call, with direct
+    launches and synchronisations between the replays."""
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    try:
+        for call, shift in enumerate([0.0, 0.05, -0.1, 0.2]):
+            reqs = _per_sim_requests(shift)
+            _core.core.set_tuning("group_graph", 1)
+            graphed = _core.core.run_groups([cases.copy_args(r) for r in reqs])
+            info = _core.core.run_groups_info()
+            assert info["graph"] == (1 if call == 0 else 2), (call, info)
+            _core.core.set_tuning("group_graph", 0)
+            direct = _core.core.run_groups([cases.copy_args(r) for r in reqs])
+            assert _core.core.run_groups_info()["graph"] == 0
+            for g, d in zip(graphed, direct):
+                for k in ["sim_bold", "sim_fc_trils", "sim_fcd_trils", "_sim_states"]:
+                    np.testing.assert_array_equal(g[k], d[k], err_msg=f"call {call} {k}")
+            _core.core.set_tuning("group_graph", 1)
+    finally:
+        _core.core.set_tuning("group_graph", 1)
+
+
+def test_autotuner_times_shapes_and_caches(tmp_path, monkeypatch):
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("CUBNM_AMD_TUNE_CACHE", str(cache))
+    reqs = _per_sim_requests(0.0)
+    _core.core.set_tuning("autotune", 1)
+    try:
+        first = _core.core.run_groups([cases.copy_args(r) for r in reqs])
+        info = _core.core.run_groups_info()
+        assert info["tuned"] in (1, 2) and info["ksplit"] >= 1
+        if info["tuned"] == 1:
+            assert info["shapes_timed"] >= 2 and info["trial_ms"] > 0
+            assert cache.exists() and "general" in cache.read_text()
+        second = _core.core.run_groups([cases.copy_args(r) for r in reqs])
+        assert _core.core.run_groups_info()["tuned"] == 2
+    finally:
+        _core.core.set_tuning("autotune", 0)
+    for r, a, b in zip(reqs, first, second):
+        np.testing.assert_array_equal(a["sim_bold"], b["sim_bold"])
+        ref = cases.run_oracle(cases.copy_args(r))
+        tol = 1e-7 if r[0] == "Kuramoto" else 1e-9
+        assert cases.bold_rel_err(a["sim_bold"], ref["sim_bold"]).max() <= tol, r[0]
