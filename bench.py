@@ -279,7 +279,7 @@ def run_c3(args):
     Ns = rng.choice([68, 100, 200, 400], S)
     durs = rng.choice([60, 120, 300, 600], S)
     models = rng.choice(["rWW", "JR", "Kuramoto"], S, p=[0.7, 0.15, 0.15])
-    reqs, work, sc_bytes = [], 0.0, 0.0
+    reqs, work, sc_bytes, coupling_flops = [], 0.0, 0.0, 0.0
     for m in ["rWW", "JR", "Kuramoto"]:
         for n in [68, 100, 200, 400]:
             for d in [60, 120, 300, 600]:
@@ -294,10 +294,13 @@ def run_c3(args):
                 reqs.append(cases.sim_args(m, scs, gp, rp, cfg, k, duration_s=d, TR_s=0.72,
                                            sc_indices=np.arange(k, dtype=np.intc)))
                 work += k * n * d
-                # algorithmic HBM bytes of the integration: every step reads
-                # the simulation's SC (N^2 doubles); SCs that fit the launch's
-                # 64 KB LDS budget (N=68 here) are read once and not counted
-                if n > 68:
+                # coupling FP64 ops: N multiply-adds per node and model step
+                coupling_flops += 2.0 * n * n * (d * 10000) * k
+                # SC bytes that must stream every model step: an SC of N=200
+                # (320 KB) or N=400 (1.28 MB) exceeds a CU's 160 KB of LDS, so
+                # no design keeps it on chip between steps; N=68 and N=100
+                # (37 and 80 KB) could stay in LDS and are not counted
+                if n >= 200:
                     sc_bytes += 8.0 * n * n * (d * 10000) * k
     log(f"c3: {S} simulations in {len(reqs)} requests, {work:.3e} sim-node-s")
     _core.core.run_groups(reqs[:1])  # session and noise set-up for the first bucket (untimed)
@@ -308,6 +311,8 @@ def run_c3(args):
     setup = sum(o["init_time"] for o in outs)  # host noise generation and session set-up
     int_ms = max(o["integrate_ms"] for o in outs)  # the one per-simulation launch
     achieved = sc_bytes / (int_ms / 1000) / 1e9
+    info = _core.core.run_groups_info()
+    log(f"c3: launch shape {info}")
     line = {
         "metric": METRIC, "value": round(work / wall, 1), "unit": "sim-node-s/s", "n_gpus": 1, "steps": 1,
         "warmup": 0, "ms_per_step": round(1000 * wall, 1), "higher_is_better": True, "scaling": "weak",
@@ -316,11 +321,17 @@ def run_c3(args):
         "config": {"workload": f"heterogeneous batch: {S} sims, N in {{68,100,200,400}}, 60-600 s, rWW/JR/Kuramoto "
                                f"70/15/15, own SC each, TR 0.72 s, FCD 30/5 TRs, {len(reqs)} requests in one call",
                    "sim_node_seconds": work, "host_setup_s": round(setup, 2),
-                   "value_excluding_host_setup": round(work / (wall - setup), 1)},
+                   "value_excluding_host_setup": round(work / (wall - setup), 1),
+                   "launch_shape": {"ksplit": int(info["ksplit"]), "pair": int(info["pair"]),
+                                    "chosen_by": ["rule", "timed trials", "on-disk cache"][int(info["tuned"])],
+                                    "trial_ms": round(info["trial_ms"], 1), "shapes_timed": int(info["shapes_timed"]),
+                                    "graph": ["none", "captured", "replayed"][int(info["graph"])]},
+                   "coupling_fp64_tflops": round(coupling_flops / (int_ms / 1000) / 1e12, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "traffic": None,
                      "kernel": "integrate_general_multi_kernel", "kernel_ms": round(int_ms, 1),
-                     "algorithmic_bytes": sc_bytes},
+                     "algorithmic_bytes": sc_bytes,
+                     "bytes_counted": "SC of the N=200 and N=400 simulations, read once per model step"},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -331,8 +342,8 @@ def run_c5(args):
     other parameters at their defaults, FIC off), 300 s at dt 0.1 ms, TR 0.72 s
     (416 volumes), FC and FCD over windows of 30 TRs with step 1 and no burn-in
     (the reference's window placement, utils.cpp:21-72), FC/FCD as split-BF16
-    Gram products (fc_mode 2), integration FP64 on the per-simulation
-    integrator (the batched one stops at 256 nodes).  --c5-sims simulations
+    Gram products (fc_mode 2), integration FP64 on the batched MFMA
+    integrator for 257..512 nodes (SC fragments streamed from L2).  --c5-sims simulations
     (8192 in the full configuration).  value = sim-node-seconds per wall
     second of the whole run_simulations call.  For the first --c5-check-sims
     simulations the FC and FCD are recomputed on the host in FP64 (NumPy
@@ -362,6 +373,7 @@ def run_c5(args):
         wall = time.time() - t0
         t = _core.last_timings()
         assert t["fc_path"] == 2
+        integrator = ["per-simulation", "batched MFMA (wide)", "many-node"][int(t["integrator"])]
     finally:
         _core.core.set_tuning("fc_mode", 0)
     vols = out["sim_bold"].shape[1] // N
@@ -391,12 +403,15 @@ def run_c5(args):
                    "fcd_bf16x3_tflops_lower_bound": round(fcd_flops / (t["fc_fcd_ms"] / 1000) / 1e12, 1),
                    "check_sims": C, "fc_max_abs_err_vs_fp64_host": fc_err,
                    "fcd_max_abs_err_vs_fp64_host": fcd_err},
-        # the dominant kernel is the per-simulation integrator: algorithmic
-        # FP64 ops (rWW, N=400) over its launch time
-        "roofline": {"bound": "mfma", "achieved": round(int_flops / (t["integrate_ms"] / 1000) / 1e12, 3),
+        # the dominant kernel is the integrator: algorithmic FP64 ops (rWW,
+        # N=400: coupling on the matrix cores, model and BW steps on the
+        # vector units, one FP64 peak for both) over its launch time
+        "roofline": {"bound": "mfma" if t["integrator"] == 1 else "fp64-vector",
+                     "achieved": round(int_flops / (t["integrate_ms"] / 1000) / 1e12, 3),
                      "peak": 78.6, "unit": "TFLOP/s",
                      "frac": round(int_flops / (t["integrate_ms"] / 1000) / 1e12 / 78.6, 4), "traffic": None,
-                     "kernel": "integrate_general_kernel", "kernel_ms": round(t["integrate_ms"], 1)},
+                     "kernel": ("integrate_batched_wide_kernel" if t["integrator"] == 1 else "integrate_general_kernel"),
+                     "integrator": integrator, "kernel_ms": round(t["integrate_ms"], 1)},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

@@ -32,8 +32,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["session.cpp", "integrate_general.hip", "integrate_batched.hip", "fc_fcd.hip", "fc_gram.hip",
-               "gof.hip", "adjoint.hip", "loss.hip"]
+HIP_SOURCES = ["session.cpp", "integrate_general.hip", "integrate_batched.hip", "integrate_batched_wide.hip",
+               "fc_fcd.hip", "fc_gram.hip", "gof.hip", "adjoint.hip", "loss.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are disabled for all device code

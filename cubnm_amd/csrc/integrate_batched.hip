@@ -328,15 +328,28 @@ hipError_t dispatch(const BatchedArgs& ba, const void* consts, int tpw, hipStrea
 
 int g_batched_tpw = 0;  // 0 = heuristic; set by the tuning entry point
 
+bool batched_wide_supported(int model, int nodes);
+int batched_wide_grid(int n_groups);
+hipError_t launch_integrate_batched_wide(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
+                                         const int* d_group_sims, int n_groups, const double* d_afrag, int KT,
+                                         double* d_lanes, int grid);
+
+// Up to 256 nodes every model (SC fragments in LDS or streamed, tiles per
+// wave 1 or 2); 257..512 nodes the conductance models with at most two
+// dynamic states per node (integrate_batched_wide.hip).
 bool batched_supported(int model, int nodes) {
-    (void)model;
-    return nodes >= 1 && nodes <= 256;
+    return (nodes >= 1 && nodes <= 256) || batched_wide_supported(model, nodes);
 }
 
 // Tiles per wave and the padded tile count (waves * TPW) the fragment array
 // and the kernel use; every wave then owns exactly TPW allocated tiles.
 void batched_plan(int nodes, int want_tpw, int* tpw, int* nt_pad) {
     const int nt = (nodes + 15) / 16;
+    if (nodes > 256) {  // wide integrator: tiles dealt round-robin to 8 waves, no padding tiles
+        *tpw = 0;
+        *nt_pad = nt;
+        return;
+    }
     int t = want_tpw != 0 ? want_tpw : g_batched_tpw;
     if (t != 1 && t != 2) t = (nt > 8) ? 2 : 1;
     const int waves = (nt + t - 1) / t;
@@ -345,8 +358,13 @@ void batched_plan(int nodes, int want_tpw, int* tpw, int* nt_pad) {
 }
 
 // Source-node fragments per tile: ceil(nodes/4) rounded up to even (the
-// coupling loop consumes fragments in pairs; padded fragments are zero).
-int batched_kt(int nodes) { return (((nodes + 3) / 4) + 1) & ~1; }
+// coupling loop consumes fragments in pairs), to a multiple of 4 above 256
+// nodes (the wide integrator's loop consumes them in fours); padded fragments
+// are zero.
+int batched_kt(int nodes) {
+    const int kt = (nodes + 3) / 4;
+    return nodes > 256 ? (kt + 3) & ~3 : (kt + 1) & ~1;
+}
 
 size_t batched_afrag_doubles(int nodes, int nt_pad, int n_scs) {
     const int KT = batched_kt(nodes);
@@ -363,7 +381,12 @@ hipError_t build_afrag(const double* d_SC, int n_scs, int nodes, int nt_pad, dou
 
 hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
                                     int* launches, const int* d_group_sims, int n_groups, const double* d_afrag,
-                                    int want_tpw) {
+                                    int want_tpw, double* d_lanes) {
+    if (a.nodes > 256) {
+        *launches = 1;
+        return launch_integrate_batched_wide(model, a, consts, s, d_group_sims, n_groups, d_afrag, batched_kt(a.nodes),
+                                             d_lanes, batched_wide_grid(n_groups));
+    }
     BatchedArgs ba;
     ba.base = a;
     ba.group_sims = d_group_sims;

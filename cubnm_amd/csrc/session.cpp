@@ -38,8 +38,10 @@ namespace cubnm {
 
 hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
                                     int* launches, const int* d_group_sims, int n_groups, const double* d_afrag,
-                                    int want_tpw);
+                                    int want_tpw, double* d_lanes);
 bool batched_supported(int model, int nodes);
+int batched_wide_grid(int n_groups);
+size_t batched_wide_lane_doubles(int model, int nodes, int grid);
 void batched_plan(int nodes, int want_tpw, int* tpw, int* nt_pad);
 size_t batched_afrag_doubles(int nodes, int nt_pad, int n_scs);
 hipError_t build_afrag(const double* d_SC, int n_scs, int nodes, int nt_pad, double* d_afrag, hipStream_t s);
@@ -171,7 +173,7 @@ struct Session {
     // device buffers
     DevBuf d_shn, d_sht, d_pairs, d_wins, d_wpairs;
     DevBuf d_SC, d_SCi, d_gp, d_rp, d_v, d_bold, d_fc, d_fcd, d_states, d_gb, d_gi, d_delays, d_maxd, d_hist, d_wfc;
-    DevBuf d_groups, d_afrag, d_pair_of, d_gram;
+    DevBuf d_groups, d_afrag, d_pair_of, d_gram, d_lanes;
     std::vector<int> groups_h;
     int last_fc_path = -1;    // 0 LDS kernel, 1 FP64 Gram, 2 split-BF16 Gram
     std::vector<unsigned char> gb_host;   // staged FIC flags (alive until the run completes)
@@ -615,6 +617,9 @@ IntegrateArgs setup_integrate(Session& s, const DevIO& io, const std::vector<int
         batched_plan(N, s.tpw, &tpw, &nt_pad);
         s.d_afrag.ensure(sizeof(double) * batched_afrag_doubles(N, nt_pad, s.N_SCs));
         check(build_afrag(io.SC, s.N_SCs, N, nt_pad, s.d_afrag.as<double>(), s.stream), "build_afrag launch");
+        if (N > 256)  // lane-ordered parameters and BW states of the wide integrator
+            s.d_lanes.ensure(sizeof(double) *
+                             batched_wide_lane_doubles(s.model->id, N, batched_wide_grid(s.n_groups)));
     }
     return a;
 }
@@ -626,7 +631,7 @@ void launch_integrator(Session& s, const IntegrateArgs& a) {
         check(launch_integrate_wide(s.model->id, a, model_consts(s), s.stream), "integrate_wide launch");
     else if (s.last_integrator == 1)
         check(launch_integrate_batched(s.model->id, a, model_consts(s), s.stream, &launches, s.d_groups.as<int>(),
-                                       s.n_groups, s.d_afrag.as<double>(), s.tpw),
+                                       s.n_groups, s.d_afrag.as<double>(), s.tpw, s.d_lanes.as<double>()),
               "integrate_batched launch");
     else check(launch_integrate_general(s.model->id, a, model_consts(s), s.stream,
                                         s.N_SIMS <= 512 || s.N_SIMS >= 4 * s.N_SCs),
