@@ -101,8 +101,8 @@ def test_odd_nodes_with_interhemispheric_exclusion_raises():
 
 def test_too_many_nodes_raises():
     import cubnm_amd.core as core
-    a = _args(nodes=1025)
-    with pytest.raises(RuntimeError, match="1024"):
+    a = _args(nodes=8193)
+    with pytest.raises(RuntimeError, match="8192"):
         core.run_simulations(*a)
 
 
