@@ -9,9 +9,13 @@ TRs step 2, 30 s burn-in, analytical FIC off.  SC: seeded lognormal(0,1),
 symmetrised, zero diagonal, max 1.  Empirical target: seeded AR(1) BOLD
 100x60 (phi 0.7, Cholesky-correlated innovations from a Wishart covariance).
 
-One step = integrate all simulations (BOLD, states), FC + FCD, and the
-FC-Pearson and FCD-KS scores against the empirical target, with inputs and
-outputs resident in HBM (cubnm_run_device).  Prints ONE JSON line.
+The grid is processed in batches of --batch simulations (default 8,192, a
+quarter of the grid); one step = one batch: integrate its simulations (BOLD,
+states), FC + FCD, and the FC-Pearson and FCD-KS scores against the empirical
+target, with inputs and outputs resident in HBM (cubnm_run_device).  Step i
+runs batch i mod (grid / batch), so every run of a multiple of 4 steps covers
+the whole grid an equal number of times; every batch is a full 60 s
+integration of its simulations.  Prints ONE JSON line.
 """
 import argparse
 import json
@@ -126,9 +130,11 @@ def read_traffic(path):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--sims", type=int, default=32768)
+    ap.add_argument("--batch", type=int, default=8192,
+                    help="simulations per step (a divisor of --sims; the grid is run batch by batch)")
     ap.add_argument("--nodes", type=int, default=100)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--cpu-cores", type=int, default=0,
@@ -168,7 +174,10 @@ def main():
         return run_c5(args)
 
     dev = torch.device("cuda:0")
-    S, N, dur = args.sims, args.nodes, args.duration
+    S, N, dur, B = args.sims, args.nodes, args.duration, args.batch
+    if B <= 0 or S % B != 0:
+        raise SystemExit(f"--batch {B} must divide --sims {S}")
+    n_batches = S // B
     TR_s, dt, bw_dt = 1.0, 0.1, 1.0
     sc = cases.synthetic_sc(N, seed=0)
     G, W, J = grid_params(S, N)
@@ -179,29 +188,40 @@ def main():
 
     capi.set_tuning("batched_tpw", args.tpw)
     capi.force_integrator(-1)
-    d = capi.prepare("rWW", cfg, S, N, int(dur * 1000), int(TR_s * 1000), int(TR_s * 1000), 0, dt, bw_dt)
-    gp, rp = cases.params("rWW", S, N, G=G, w_p=W[:, None], J_N=J[:, None], do_fic=False, wIE=1.0)
+    d = capi.prepare("rWW", cfg, B, N, int(dur * 1000), int(TR_s * 1000), int(TR_s * 1000), 0, dt, bw_dt)
     t = lambda a, dtype=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
-    d_sc, d_sci = t(sc.reshape(1, -1)), t(np.zeros(S), torch.int32)
-    d_dist, d_gp, d_rp, d_v = t(np.zeros(N * N)), t(gp), t(rp), t(np.zeros(S))
+    # every batch's parameters resident in HBM ([param][sim] and
+    # [param][sim][node] per batch), outputs for the whole grid
+    d_gp, d_rp = [], []
+    for b in range(n_batches):
+        sl = slice(b * B, (b + 1) * B)
+        gp, rp = cases.params("rWW", B, N, G=G[sl], w_p=W[sl][:, None], J_N=J[sl][:, None], do_fic=False, wIE=1.0)
+        d_gp.append(t(gp))
+        d_rp.append(t(rp))
+    d_sc, d_sci = t(sc.reshape(1, -1)), t(np.zeros(B), torch.int32)
+    d_dist, d_v = t(np.zeros(N * N)), t(np.zeros(B))
     d_bold = torch.empty((S, d.bold_len * N), dtype=torch.float64, device=dev)
     d_fc = torch.empty((S, d.n_pairs), dtype=torch.float64, device=dev)
     d_fcd = torch.empty((S, d.n_window_pairs), dtype=torch.float64, device=dev)
-    d_st = torch.empty((d.n_state_vars, S, N), dtype=torch.float64, device=dev)
-    d_gb = torch.zeros((d.n_global_out_bool, S), dtype=torch.uint8, device=dev)
-    d_gi = torch.zeros((d.n_global_out_int, S), dtype=torch.int32, device=dev)
+    d_st = torch.empty((n_batches, d.n_state_vars, B, N), dtype=torch.float64, device=dev)
+    d_gb = torch.zeros((n_batches, d.n_global_out_bool, B), dtype=torch.uint8, device=dev)
+    d_gi = torch.zeros((n_batches, d.n_global_out_int, B), dtype=torch.int32, device=dev)
     d_emp_fc, d_emp_fcd = t(emp_fc), t(emp_fcd)
-    d_corr = torch.empty(S, dtype=torch.float64, device=dev)
-    d_ks = torch.empty(S, dtype=torch.float64, device=dev)
+    d_corr = torch.full((S,), float("nan"), dtype=torch.float64, device=dev)
+    d_ks = torch.full((S,), float("nan"), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+    counter = [0]
 
     def step():
-        capi.run_device(d_sc.data_ptr(), d_sci.data_ptr(), d_dist.data_ptr(), d_gp.data_ptr(), d_rp.data_ptr(),
-                        d_v.data_ptr(), d_bold.data_ptr(), d_fc.data_ptr(), d_fcd.data_ptr(), d_st.data_ptr(),
-                        d_gb.data_ptr(), d_gi.data_ptr())
-        capi.gof_fc_corr_device(d_fc.data_ptr(), S, d.n_pairs, d_emp_fc.data_ptr(), d_corr.data_ptr())
-        capi.gof_fcd_ks_device(d_fcd.data_ptr(), S, d.n_window_pairs, d_emp_fcd.data_ptr(), emp_fcd.size,
-                               d_ks.data_ptr())
+        b = counter[0] % n_batches
+        counter[0] += 1
+        o = b * B
+        capi.run_device(d_sc.data_ptr(), d_sci.data_ptr(), d_dist.data_ptr(), d_gp[b].data_ptr(),
+                        d_rp[b].data_ptr(), d_v.data_ptr(), d_bold[o].data_ptr(), d_fc[o].data_ptr(),
+                        d_fcd[o].data_ptr(), d_st[b].data_ptr(), d_gb[b].data_ptr(), d_gi[b].data_ptr())
+        capi.gof_fc_corr_device(d_fc[o].data_ptr(), B, d.n_pairs, d_emp_fc.data_ptr(), d_corr[o].data_ptr())
+        capi.gof_fcd_ks_device(d_fcd[o].data_ptr(), B, d.n_window_pairs, d_emp_fcd.data_ptr(), emp_fcd.size,
+                               d_ks[o].data_ptr())
         return capi.last_timings()
 
     if args.copy_calibration_mib:
@@ -227,12 +247,14 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    value = S * N * dur / (ms_per_step / 1000.0)
+    value = B * N * dur / (ms_per_step / 1000.0)
 
     corr, ks = d_corr.cpu().numpy(), d_ks.cpu().numpy()
-    log(f"scores: fc_corr mean {np.nanmean(corr):.4f}, fcd_ks mean {np.nanmean(ks):.4f}, "
-        f"nan {int(np.isnan(corr).sum())}/{int(np.isnan(ks).sum())}")
-    ops = ops_per_sim_node_second(N, dt, bw_dt, TR_s) * S * N * dur
+    scored = min(S, counter[0] * B)
+    log(f"scores of the first {scored} simulations: fc_corr mean {np.nanmean(corr[:scored]):.4f}, "
+        f"fcd_ks mean {np.nanmean(ks[:scored]):.4f}, "
+        f"nan {int(np.isnan(corr[:scored]).sum())}/{int(np.isnan(ks[:scored]).sum())}")
+    ops = ops_per_sim_node_second(N, dt, bw_dt, TR_s) * B * N * dur
     k_ms = float(np.mean(integ_ms))
     achieved = ops / (k_ms / 1000.0) / 1e12
     traffic = read_traffic(os.path.join(ROOT, "profiles", "integrate_batched_rww_n100_c2_traffic.txt"))
@@ -254,8 +276,10 @@ def main():
         "vs_baseline": round(value / A100_PUBLISHED, 3),
         "dtype": "f64", "data": "synthetic (seeded lognormal SC, seeded AR(1) empirical BOLD, host-generated noise)",
         "config": {"workload": f"rWW grid {S} sims x {N} nodes x {dur:g} s (dt 0.1 ms, BW dt 1 ms, TR 1 s, "
-                               f"FCD window 10 step 2, FIC off), FC + FCD + FC-Pearson + FCD-KS on GPU",
-                   "sims": S, "nodes": N, "duration_s": dur, "integrator": "batched MFMA FP64"},
+                               f"FCD window 10 step 2, FIC off), FC + FCD + FC-Pearson + FCD-KS on GPU, "
+                               f"{B} simulations per step ({n_batches} steps per pass over the grid)",
+                   "sims": S, "sims_per_step": B, "nodes": N, "duration_s": dur,
+                   "integrator": "batched MFMA FP64"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
