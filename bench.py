@@ -154,6 +154,7 @@ def main():
                     help="Adam learning rate in the unit cube (c4; 0.12 was the best of 0.02, 0.05, 0.1, 0.12, "
                          "0.15, 0.2, 0.3)")
     ap.add_argument("--c5-sims", type=int, default=8192, help="simulations (c5)")
+    ap.add_argument("--c5-batch", type=int, default=4096, help="simulations per run_simulations call (c5)")
     ap.add_argument("--c5-check-sims", type=int, default=4,
                     help="simulations whose FC/FCD are recomputed in FP64 on the host (c5)")
     args = ap.parse_args()
@@ -365,50 +366,63 @@ def run_c5(args):
     """N=400 rWW (one shared seeded lognormal SC, G uniform in [0.5, 3],
     other parameters at their defaults, FIC off), 300 s at dt 0.1 ms, TR 0.72 s
     (416 volumes), FC and FCD over windows of 30 TRs with step 1 and no burn-in
-    (the reference's window placement, utils.cpp:21-72), FC/FCD as split-BF16
-    Gram products (fc_mode 2), integration FP64 on the batched MFMA
-    integrator for 257..512 nodes (SC fragments streamed from L2).  --c5-sims simulations
-    (8192 in the full configuration).  value = sim-node-seconds per wall
-    second of the whole run_simulations call.  For the first --c5-check-sims
-    simulations the FC and FCD are recomputed on the host in FP64 (NumPy
-    Pearson correlations of the returned BOLD, cubnm_amd.utils) and the
-    largest differences are reported."""
+    (the reference's window placement, utils.cpp:21-72: windows centred on a
+    volume span window_size + 1 = 31 volumes, 386 of them; 30-volume windows
+    are not expressible in that rule), FC/FCD as split-BF16 Gram products
+    (fc_mode 2), integration FP64 on the batched MFMA integrator for
+    257..512 nodes (SC fragments streamed from L2).  --c5-sims simulations
+    (8192 in the full configuration) in run_simulations calls of --c5-batch
+    (4096: 256 workgroups of 16 simulations, one per CU).  value =
+    sim-node-seconds per wall second of the calls.  For the first
+    --c5-check-sims simulations the FC and FCD are recomputed on the host in
+    FP64 (NumPy Pearson correlations of the returned BOLD, cubnm_amd.utils)
+    and the largest differences are reported."""
     import torch  # noqa: F401
     import cases
     from cubnm_amd import _core, utils
     N, dur, TR = 400, 300.0, 0.72
     sc = cases.synthetic_sc(N, seed=0)
-    S = args.c5_sims
+    S, B = args.c5_sims, min(args.c5_batch, args.c5_sims)
     G = np.random.default_rng(5).uniform(0.5, 3.0, S)
 
-    def args_for(n, d):
-        gp, rp = cases.params("rWW", n, N, G=G[:n], do_fic=False, wIE=1.0)
+    def args_for(g, d):
+        n = g.size
+        gp, rp = cases.params("rWW", n, N, G=g, do_fic=False, wIE=1.0)
         cfg = cases.model_config("rWW", TR=TR, bold_remove_s=0, do_fic=False)
         cfg.update({"window_size": "30", "window_step": "1", "verbose": "0"})
         return cases.sim_args("rWW", sc, gp, rp, cfg, n, duration_s=d, TR_s=TR)
 
     _core.core.set_tuning("fc_mode", 2)
-    try:
-        log(f"c5: warm-up (noise segment, workspaces): 16 simulations of 30 s")
-        _core.run_simulations(*args_for(16, 30.0))
-        log(f"c5: {S} simulations of {dur:g} s")
-        t0 = time.time()
-        out = _core.run_simulations(*args_for(S, dur))
-        wall = time.time() - t0
-        t = _core.last_timings()
-        assert t["fc_path"] == 2
-        integrator = ["per-simulation", "batched MFMA (wide)", "many-node"][int(t["integrator"])]
-    finally:
-        _core.core.set_tuning("fc_mode", 0)
-    vols = out["sim_bold"].shape[1] // N
+    wall = integ_ms = fc_ms = 0.0
     C = min(args.c5_check_sims, S)
     fc_err = fcd_err = 0.0
-    for i in range(C):
-        b = out["sim_bold"][i].reshape(vols, N).T
-        fc_err = max(fc_err, float(np.abs(utils.calculate_fc(b) - out["sim_fc_trils"][i]).max()))
-        fcd_err = max(fcd_err, float(np.abs(utils.calculate_fcd(b, 30, 1) - out["sim_fcd_trils"][i]).max()))
-        log(f"c5: check {i + 1}/{C}: FC err {fc_err:.2e}, FCD err {fcd_err:.2e}")
-    n_w = int(round((1 + np.sqrt(1 + 8 * out["sim_fcd_trils"].shape[1])) / 2))
+    try:
+        log(f"c5: warm-up (noise segment, workspaces): 16 simulations of 30 s")
+        _core.run_simulations(*args_for(G[:16], 30.0))
+        for b0 in range(0, S, B):
+            g = G[b0:b0 + B]
+            t0 = time.time()
+            out = _core.run_simulations(*args_for(g, dur))
+            dt_call = time.time() - t0
+            wall += dt_call
+            t = _core.last_timings()
+            assert t["fc_path"] == 2
+            integ_ms += t["integrate_ms"]
+            fc_ms += t["fc_fcd_ms"]
+            integrator = ["per-simulation", "batched MFMA (wide)", "many-node"][int(t["integrator"])]
+            log(f"c5: simulations {b0}..{b0 + g.size - 1} in {dt_call:.1f} s (integration {t['integrate_ms'] / 1000:.1f} s, "
+                f"FC/FCD {t['fc_fcd_ms'] / 1000:.2f} s, {integrator})")
+            vols = out["sim_bold"].shape[1] // N
+            n_fcd = out["sim_fcd_trils"].shape[1]
+            for i in range(min(C - b0, g.size)):
+                bb = out["sim_bold"][i].reshape(vols, N).T
+                fc_err = max(fc_err, float(np.abs(utils.calculate_fc(bb) - out["sim_fc_trils"][i]).max()))
+                fcd_err = max(fcd_err, float(np.abs(utils.calculate_fcd(bb, 30, 1) - out["sim_fcd_trils"][i]).max()))
+                log(f"c5: check {b0 + i + 1}/{C}: FC err {fc_err:.2e}, FCD err {fcd_err:.2e}")
+            del out
+    finally:
+        _core.core.set_tuning("fc_mode", 0)
+    n_w = int(round((1 + np.sqrt(1 + 8 * n_fcd)) / 2))
     pairs = N * (N - 1) // 2
     # FCD Gram products alone, lower triangle (the FCD tril), 3 BF16 terms: a
     # lower bound on the FC/FCD stage's rate
@@ -416,26 +430,31 @@ def run_c5(args):
     work = S * N * dur
     int_flops = work * ops_per_sim_node_second(N, 0.1, 1.0, TR)
     line = {
-        "metric": METRIC, "value": round(work / wall, 1), "unit": "sim-node-s/s", "n_gpus": 1, "steps": 1,
-        "warmup": 1, "ms_per_step": round(1000 * wall, 1), "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": round(work / wall, 1), "unit": "sim-node-s/s", "n_gpus": 1,
+        "steps": (S + B - 1) // B, "warmup": 1, "ms_per_step": round(1000 * wall / ((S + B - 1) // B), 1),
+        "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64 integration, split-bf16 FC/FCD products",
         "data": "synthetic (seeded lognormal SC, host-generated noise)",
-        "config": {"workload": f"rWW N={N}, {S} sims, {dur:g} s, TR {TR} s ({vols} volumes), FC + FCD over "
-                               f"{n_w} windows of 30 TRs step 1, split-BF16 Gram products",
-                   "integrate_ms": round(t["integrate_ms"], 1), "fc_fcd_ms": round(t["fc_fcd_ms"], 1),
-                   "fc_fcd_sims_per_s": round(S / (t["fc_fcd_ms"] / 1000), 1),
-                   "fcd_bf16x3_tflops_lower_bound": round(fcd_flops / (t["fc_fcd_ms"] / 1000) / 1e12, 1),
+        "config": {"workload": f"rWW N={N}, {S} sims in calls of {B}, {dur:g} s, TR {TR} s ({vols} volumes), "
+                               f"FC + FCD over {n_w} windows of 31 volumes (window_size 30) step 1, split-BF16 "
+                               f"Gram products",
+                   "integrate_ms": round(integ_ms, 1), "fc_fcd_ms": round(fc_ms, 1),
+                   "fc_fcd_sims_per_s": round(S / (fc_ms / 1000), 1),
+                   "fcd_bf16x3_tflops_lower_bound": round(fcd_flops / (fc_ms / 1000) / 1e12, 1),
                    "check_sims": C, "fc_max_abs_err_vs_fp64_host": fc_err,
                    "fcd_max_abs_err_vs_fp64_host": fcd_err},
         # the dominant kernel is the integrator: algorithmic FP64 ops (rWW,
         # N=400: coupling on the matrix cores, model and BW steps on the
-        # vector units, one FP64 peak for both) over its launch time
-        "roofline": {"bound": "mfma" if t["integrator"] == 1 else "fp64-vector",
-                     "achieved": round(int_flops / (t["integrate_ms"] / 1000) / 1e12, 3),
+        # vector units, one FP64 datapath and peak for both) over its launch
+        # time
+        "roofline": {"bound": "mfma" if integrator.startswith("batched") else "fp64-vector",
+                     "fp64_datapath": "FP64 MFMA and FP64 VALU share one datapath: one 78.6 TFLOP/s peak for both",
+                     "achieved": round(int_flops / (integ_ms / 1000) / 1e12, 3),
                      "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": round(int_flops / (t["integrate_ms"] / 1000) / 1e12 / 78.6, 4), "traffic": None,
-                     "kernel": ("integrate_batched_wide_kernel" if t["integrator"] == 1 else "integrate_general_kernel"),
-                     "integrator": integrator, "kernel_ms": round(t["integrate_ms"], 1)},
+                     "frac": round(int_flops / (integ_ms / 1000) / 1e12 / 78.6, 4), "traffic": None,
+                     "kernel": ("integrate_batched_wide_kernel" if integrator.startswith("batched")
+                                else "integrate_general_kernel"),
+                     "integrator": integrator, "kernel_ms": round(integ_ms, 1)},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
