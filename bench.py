@@ -110,7 +110,7 @@ def cpu_baseline(n_cores, nodes, duration_s, G, W, J, sc, cfg, per_core=2, repea
         "cores": n_cores,
         "kind": "port",
         "cpu_model": model,
-        "sample": f"{n} of the grid's simulations (evenly spaced), full {duration_s:g} s each, "
+        "sample": f"{n} of the workload's simulations (evenly spaced), full {duration_s:g} s each, "
                   f"{per_core} per OpenMP thread on {n_cores} threads; run phase only, noise generation "
                   f"excluded; median of {repeats} runs ({', '.join(f'{r:.2f}' for r in rts)} s)",
     }
@@ -496,6 +496,38 @@ def run_c4(args):
     h_gd = fit.adam_fit(fit_gd, args.c4_steps, lr=args.c4_lr, seed=2, target=best, log=log)
     reached = [t for t, b in h_gd if b <= best]
     t_gd = reached[0] if reached else None
+    # one forward and one backward pass of the 256-start batch, timed alone:
+    # the adjoint kernels' share of a gradient step and their FP64 rate
+    # (algorithmic operations: the forward's 2.444e6-style count per
+    # sim-node-second, twice for the backward = recomputation + adjoint sweep
+    # of about the forward's size)
+    prob = fit_gd.p
+    rng = np.random.default_rng(7)
+    Gt = torch.tensor(rng.uniform(0.5, 3.0, 256), dtype=torch.float64, device="cuda")
+    wpt = torch.tensor(rng.uniform(0.5, 1.5, (256, N)), dtype=torch.float64, device="cuda")
+    jnt = torch.tensor(rng.uniform(0.1, 0.3, (256, N)), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.time()
+    bold = prob.forward(Gt, wpt, jnt)
+    torch.cuda.synchronize()
+    t1 = time.time()
+    prob.backward(Gt, wpt, jnt, torch.ones_like(bold))
+    torch.cuda.synchronize()
+    t2 = time.time()
+    fwd_ops = 256 * N * dur * ops_per_sim_node_second(N, 0.1, 1.0, TR_s)
+    bwd_tflops = 2 * fwd_ops / (t2 - t1) / 1e12
+    log(f"c4: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s for 256 starts")
+    cpu = None
+    if not args.no_cpu_baseline:
+        n_cores = args.cpu_cores or host_cpu()[0]
+        g = np.random.default_rng(8)
+        cfg = cases.model_config("rWW", TR=TR_s, window_size=42, window_step=7, bold_remove_s=30, do_fic=False)
+        cpu = cpu_baseline(n_cores, N, dur, g.uniform(0.5, 3.0, 64), g.uniform(0.5, 1.5, 64), g.uniform(0.1, 0.3, 64),
+                           sc, cfg, per_core=1, repeats=1)
+        cpu["cmaes_generation_s"] = round(64 * N * dur / cpu["value"], 1)
+        cpu["note"] = ("the CMA-ES generation's forward work (64 simulations of 120 s) on the host engine; "
+                       "the GPU's generation takes cmaes_seconds_to_best / generations")
+        log(f"cpu baseline: {cpu}")
     line = {
         "metric": METRIC, "value": round(t_cma / t_gd, 3) if t_gd else 0.0, "unit": "x (CMA-ES time / gradient time to CMA-ES best loss)",
         "n_gpus": 1, "steps": len(h_gd), "warmup": 0, "ms_per_step": round(1000 * h_gd[-1][0] / len(h_gd), 1),
@@ -506,8 +538,15 @@ def run_c4(args):
                                f"gradients",
                    "cmaes_best_loss": round(best, 6), "cmaes_seconds_to_best": round(t_cma, 2),
                    "grad_seconds_to_cmaes_best": round(t_gd, 2) if t_gd else None,
-                   "grad_best_loss": round(h_gd[-1][1], 6), "grad_steps": len(h_gd)},
-        "roofline": None, "cpu_baseline": None,
+                   "grad_best_loss": round(h_gd[-1][1], 6), "grad_steps": len(h_gd),
+                   "forward_s_256": round(t1 - t0, 3), "backward_s_256": round(t2 - t1, 3)},
+        # the adjoint kernels issue no MFMA: FP64 vector peak
+        "roofline": {"bound": "fp64", "achieved": round(bwd_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(bwd_tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "rww_adj_kernel (backward pass)", "kernel_ms": round(1000 * (t2 - t1), 1),
+                     "note": "one simulation per workgroup, 256 simulations on 256 CUs: bound by the latency "
+                             "of each step's dependent chain, not by issue"},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

@@ -66,12 +66,15 @@ __device__ inline void noise_load(const AdjArgs& a, int sh_t, int sh_j, int inne
     *n1 = a.noise[idx + 1];
 }
 
-// derivative of H(x) = u / (1 - exp(-d u)), u = a x - b, with respect to x
-__device__ inline double dH(double a, double b, double d, double x) {
+// H(x) = u / (1 - exp(-d u)) with u = a x - b, and its derivative
+// dH/dx = a (den - u d e) / den^2 with e = exp(-d u), den = 1 - e: one
+// exponential for both, divisions by the reciprocal sequence (fast_div).
+__device__ inline double rate_and_slope(double a, double b, double d, double x, double* slope) {
     const double u = a * x - b;
-    const double e = exp(-d * u);
+    const double e = exp_cw(-d * u);
     const double den = 1.0 - e;
-    return a * (den - u * d * e) / (den * den);
+    *slope = a * fast_div(den - u * d * e, den * den);
+    return fast_div(u, den);
 }
 
 // Adjoint state of one node (lambda of S_E, S_I and of the BW variables) and
@@ -122,15 +125,15 @@ __device__ inline void rww_adjoint_pre(const RWW::C& c, const double* v, double 
     const double SE = v[0], SI = v[1], g = v[2], n0 = v[3], n1 = v[4];
     const double IE = c.w_E__I_0 + wp * JN * SE + g * G * JN - wIE * SI;
     const double II = c.w_I__I_0 + JN * SE - c.w_II * SI;
-    const double aE = c.a_E * IE - c.b_E, aI = c.a_I * II - c.b_I;
-    rE = aE / (1 - exp(-c.d_E * aE));
-    const double rI = aI / (1 - exp(-c.d_I * aI));
+    double hE, hI;
+    rE = rate_and_slope(c.a_E, c.b_E, c.d_E, IE, &hE);
+    const double rI = rate_and_slope(c.a_I, c.b_I, c.d_I, II, &hI);
     const double SEs = SE + n0 * sig * c.sqrt_dt + c.dt_gamma_E * ((1 - SE) * rE) - c.dt_itau_E * SE;
     const double SIs = SI + n1 * sig * c.sqrt_dt + c.dt_gamma_I * rI - c.dt_itau_I * SI;
     muE = (SEs > 0.0 && SEs < 1.0) ? r.lSE : 0.0;
     muI = (SIs > 0.0 && SIs < 1.0) ? r.lSI : 0.0;
-    eE = muE * c.dt_gamma_E * (1 - SE) * dH(c.a_E, c.b_E, c.d_E, IE);
-    eI = muI * c.dt_gamma_I * dH(c.a_I, c.b_I, c.d_I, II);
+    eE = muE * c.dt_gamma_E * (1 - SE) * hE;
+    eI = muI * c.dt_gamma_I * hI;
     r.gG += eE * JN * g;
     r.gwp += eE * JN * SE;
     r.gJN += eE * (wp * SE + G * g) + eI * SE;
@@ -398,189 +401,9 @@ __global__ void __launch_bounds__(MAXT) rww_adj_kernel(AdjArgs a, RWW::C c, BWCo
     store_gradients(a, sim, j, act, blockDim.x, r, red);
 }
 
-// The backward pass with each segment's recomputation overlapped with the
-// adjoint sweep of the segment after it.  The block is two groups of ks*np
-// lanes: in phase ph (0..n_ck), group 1 recomputes segment n_ck-1-ph into
-// scratch buffer (segment & 1) while group 0 runs the adjoint backwards
-// through segment n_ck-ph from the buffer group 1 filled in the phase before.
-// Both groups pass the same three LDS barriers per inner step (the
-// barriers are anonymous, so they pair up one to one) and do independent
-// work between them; a full barrier ends each phase and makes group 1's
-// scratch writes visible to group 0.  Each group's arithmetic is that of
-// rww_adj_kernel (the same operations in the same order; the compiler may
-// contract multiply-adds differently, a difference at rounding level); the
-// recomputation, a forward pass's worth of work, runs in the latency of the
-// adjoint sweep instead of before it.
-template <int MAXT>
-__global__ void __launch_bounds__(MAXT) rww_adj_pipe_kernel(AdjArgs a, RWW::C c, BWConsts bw) {
-    extern __shared__ double lds[];
-    const int sim = blockIdx.x, N = a.N, np = a.np, ks = a.ks;
-    const int TPG = ks * np;
-    const int grp = threadIdx.x / TPG;  // 0 adjoint sweep, 1 recomputation (wave-uniform: TPG % 64 == 0)
-    const bool rev = grp == 0, rec = grp == 1;
-    const int tid = threadIdx.x - grp * TPG;
-    const int part = tid / np, j = tid - part * np;
-    const bool own = part == 0;
-    const bool act = own && j < N;
-    const int jj = j < N ? j : 0;
-    const int kc = (N + ks - 1) / ks, k0 = min(N, part * kc), k1 = min(N, k0 + kc);
-    const int ldsc = N + 1;
-    double* conn = lds;                   // [N]   recomputation: coupling state
-    double* ev = lds + N;                 // [N]   adjoint: e_j of the coupling
-    double* red = lds + 2 * N;            // [16]  block reduction
-    double* redF = red + 16;              // [ks-1][np] recomputation partial sums
-    double* redR = redF + (ks - 1) * np;  // [ks-1][np] adjoint partial sums
-    double* scs = redR + (ks - 1) * np;   // [N][N+1] SC copy when a.sc_lds
-    if (a.sc_lds)
-        for (int i = threadIdx.x; i < N * N; i += blockDim.x) scs[(i / N) * ldsc + i % N] = a.SC[i];
-    const double G = a.G[sim];
-    const double wp = a.wp[(long long)sim * N + jj], JN = a.JN[(long long)sim * N + jj];
-    const double wIE = a.wIE[(long long)sim * N + jj], sig = a.sigma[(long long)sim * N + jj];
-    const double gp[1] = {G};
-    const double rp[4] = {wp, JN, wIE, sig};
-    const int per_bw = a.inner_it * 3 + 5;
-    const long long buf_doubles = (long long)a.ck * per_bw * N;
-    double* const scr0 = a.scratch + (long long)sim * 2 * buf_doubles;  // two segment buffers
-    const double* dbold = a.dbold + (long long)sim * a.bold_len * N;
-    const double lnr1 = log(bw.oneminrho);
-    const int Tmax = a.ck * a.inner_it;
-    RevState r;
-    // recomputation state
-    double s[RWW::n_state] = {0, 0, 0, 0, 0, 0};
-    double bx = 0, bf = 1, bn = 1, bq = 1;
-    int sh_t = 0, sh_j = 0, nsh_t = 0, nsh_j = 0;
-    double pn0 = 0.0, pn1 = 0.0;
-    // adjoint-sweep state
-    double p1[5] = {0, 0, 0, 0, 0}, p2[5] = {0, 0, 0, 0, 0};
-    double pbx = 0, pbf = 1, pbn = 1, pbq = 1, pbu = 0;
-    int fb = -1, fsh_t = 0, fsh_j = 0;
-    for (int ph = 0; ph <= a.n_ck; ph++) {
-        const int segF = a.n_ck - 1 - ph, segR = a.n_ck - ph;
-        const int b0F = segF * a.ck, b1F = segF >= 0 ? min(a.bw_it, b0F + a.ck) : b0F;
-        const int b0R = segR * a.ck, b1R = segR < a.n_ck ? min(a.bw_it, b0R + a.ck) : b0R;
-        const int TF = (b1F - b0F) * a.inner_it, TR = (b1R - b0R) * a.inner_it;
-        double* const scF = scr0 + (segF & 1) * buf_doubles;
-        const double* const scR = scr0 + (segR & 1) * buf_doubles;
-        // v = S_E, S_I, g, noise 0, noise 1 of adjoint step t of segment segR
-        auto fetch = [&](int t, double* v) {
-            const int bb = b0R + t / a.inner_it, ii = t % a.inner_it;
-            if (bb != fb) { noise_shuffle(a, bb, jj, &fsh_t, &fsh_j); fb = bb; }
-            const double* sbx = scR + (long long)(bb - b0R) * per_bw * N;
-            v[0] = sbx[(3 * ii) * N + jj]; v[1] = sbx[(3 * ii + 1) * N + jj]; v[2] = sbx[(3 * ii + 2) * N + jj];
-            noise_load(a, fsh_t, fsh_j, ii, &v[3], &v[4]);
-        };
-        if (rec && TF > 0) {
-            const double* ck = a.ckpt + ((long long)sim * a.n_ck + segF) * 6 * N;
-            if (own) {
-                s[4] = ck[0 * N + jj]; s[5] = ck[1 * N + jj];
-                bx = ck[2 * N + jj]; bf = ck[3 * N + jj]; bn = ck[4 * N + jj]; bq = ck[5 * N + jj];
-                noise_shuffle(a, b0F, jj, &sh_t, &sh_j);
-                noise_load(a, sh_t, sh_j, 0, &pn0, &pn1);
-            }
-            if (act) conn[j] = s[4];
-        }
-        if (rev && TR > 0 && own) {
-            fb = -1;
-            fetch(TR - 1, p1);
-            if (TR >= 2) fetch(TR - 2, p2);
-            const double* bwst = scR + (long long)(b1R - 1 - b0R) * per_bw * N + (long long)3 * a.inner_it * N;
-            pbx = bwst[0 * N + jj]; pbf = bwst[1 * N + jj]; pbn = bwst[2 * N + jj]; pbq = bwst[3 * N + jj];
-            pbu = bwst[4 * N + jj];
-        }
-        for (int st = 0; st < Tmax; st++) {
-            const bool fon = rec && st < TF, ron = rev && st < TR;
-            // recomputation: forward step st of segment segF
-            const int fi = st % a.inner_it, fbl = st / a.inner_it;
-            double n0 = 0.0, n1 = 0.0;
-            if (fon && own) {
-                if (fi == 0) {
-                    nsh_t = sh_t; nsh_j = sh_j;
-                    if (b0F + fbl + 1 < b1F) noise_shuffle(a, b0F + fbl + 1, jj, &nsh_t, &nsh_j);
-                }
-                n0 = pn0; n1 = pn1;
-                if (fi + 1 < a.inner_it) noise_load(a, sh_t, sh_j, fi + 1, &pn0, &pn1);
-                else if (b0F + fbl + 1 < b1F) noise_load(a, nsh_t, nsh_j, 0, &pn0, &pn1);
-            }
-            // adjoint: backward step rt of segment segR
-            const int rt = TR - 1 - st;
-            const int ri = ron ? rt % a.inner_it : 0, rb = ron ? b0R + rt / a.inner_it : 0;
-            double eE = 0.0, eI = 0.0, muE = 0.0, muI = 0.0, rE = 0.0;
-            if (ron && own) {
-                if (ri == a.inner_it - 1) {
-                    const double x = pbx, f = pbf, nu = pbn, q = pbq, u = pbu;
-                    if (rb > b0R) {
-                        const double* bwst = scR + (long long)(rb - 1 - b0R) * per_bw * N + (long long)3 * a.inner_it * N;
-                        pbx = bwst[0 * N + jj]; pbf = bwst[1 * N + jj]; pbn = bwst[2 * N + jj]; pbq = bwst[3 * N + jj];
-                        pbu = bwst[4 * N + jj];
-                    }
-                    bw_adjoint(a, bw, rb, x, f, nu, q, u, dbold, jj, lnr1, r);
-                }
-                double v[5];
-#pragma unroll
-                for (int q = 0; q < 5; q++) { v[q] = p1[q]; p1[q] = p2[q]; }
-                if (rt >= 2) fetch(rt - 2, p2);
-                rww_adjoint_pre(c, v, G, wp, JN, wIE, sig, r, eE, eI, muE, muI, rE);
-            }
-            lds_barrier();  // 1
-            double g = 0.0;
-            if (fon) {
-                if (a.sc_lds) {
-#pragma unroll 8
-                    for (int k = k0; k < k1; k++) g += scs[k * ldsc + jj] * conn[k];
-                } else {
-#pragma unroll 8
-                    for (int k = k0; k < k1; k++) g += a.SC[(long long)k * N + jj] * conn[k];
-                }
-                if (ks > 1 && !own) redF[(part - 1) * np + j] = g;
-            }
-            if (ron && act) ev[j] = eE * G * JN;
-            lds_barrier();  // 2
-            if (fon && own) {
-                if (ks > 1)
-                    for (int p = 1; p < ks; p++) g += redF[(p - 1) * np + j];
-                if (act) {
-                    double* sb = scF + (long long)fbl * per_bw * N;
-                    sb[(3 * fi) * N + j] = s[4]; sb[(3 * fi + 1) * N + j] = s[5]; sb[(3 * fi + 2) * N + j] = g;
-                }
-                RWW::step(c, s, gp, rp, g, n0, n1);
-            }
-            double coup = 0.0;
-            if (ron) {
-                if (a.sc_lds) {
-#pragma unroll 8
-                    for (int m = k0; m < k1; m++) coup += scs[jj * ldsc + m] * ev[m];
-                } else {
-#pragma unroll 8
-                    for (int m = k0; m < k1; m++) coup += a.SC[(long long)jj * N + m] * ev[m];
-                }
-                if (ks > 1 && !own) redR[(part - 1) * np + j] = coup;
-            }
-            lds_barrier();  // 3
-            if (fon) {
-                if (act) conn[j] = s[4];
-                if (own && fi == a.inner_it - 1) {
-                    double* bwst = scF + (long long)fbl * per_bw * N + (long long)3 * a.inner_it * N;
-                    if (act) { bwst[0 * N + j] = bx; bwst[1 * N + j] = bf; bwst[2 * N + j] = bn; bwst[3 * N + j] = bq; bwst[4 * N + j] = s[4]; }
-                    bw_step(bw, s[4], bx, bf, bn, bq);
-                    sh_t = nsh_t; sh_j = nsh_j;
-                }
-            }
-            if (ron && own) {
-                if (ks > 1)
-                    for (int p = 1; p < ks; p++) coup += redR[(p - 1) * np + j];
-                rww_adjoint_post(c, eE, eI, muE, muI, rE, wp, JN, wIE, coup, r);
-            }
-        }
-        __syncthreads();  // end of phase: group 1's scratch and the LDS state are complete
-    }
-    store_gradients(a, sim, j, act && rev, TPG, r, red);
-}
-
 }  // namespace cubnm
 
 using namespace cubnm;
-
-int g_adjoint_pipe = 0;
 
 namespace {
 
@@ -613,13 +436,6 @@ int fill_args(AdjArgs& A, RWW::C& c, BWConsts& bw, int n_sims, int nodes, int ti
     return 0;
 }
 
-// The overlapped backward pass needs two groups of ks*np lanes in one block
-// of at most 512 (up to 256 nodes) and two segment buffers of scratch; the
-// scratch size does not depend on the adjoint_pipe knob, so buffers sized
-// before the knob changes stay large enough.
-bool pipe_fits(const AdjArgs& A) { return 2 * A.ks * A.np <= 512; }
-bool use_pipe(const AdjArgs& A) { return g_adjoint_pipe != 0 && pipe_fits(A); }
-
 int finish(hipError_t e) {
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) { set_last_error(std::string("HIP error: ") + hipGetErrorString(e)); return -1; }
@@ -638,7 +454,7 @@ int cubnm_rww_grad_sizes(int n_sims, int nodes, int time_steps, int BOLD_TR, dou
         return -1;
     *bold_doubles = (long long)n_sims * A.bold_len * nodes;
     *ckpt_doubles = (long long)n_sims * A.n_ck * 6 * nodes;
-    *scratch_doubles = (long long)n_sims * (pipe_fits(A) ? 2 : 1) * ck * (A.inner_it * 3 + 5) * nodes;
+    *scratch_doubles = (long long)n_sims * ck * (A.inner_it * 3 + 5) * nodes;
     return 0;
 }
 
@@ -673,14 +489,6 @@ int cubnm_rww_grad_backward(int n_sims, int nodes, int time_steps, int BOLD_TR, 
         return -1;
     A.ckpt = const_cast<double*>(d_ckpt); A.dbold = d_dbold; A.scratch = d_scratch;
     A.gG = d_gG; A.gwp = d_gwp; A.gJN = d_gJN;
-    if (use_pipe(A)) {
-        const size_t base = sizeof(double) * (2 * (size_t)nodes + 16 + 2 * (size_t)(A.ks - 1) * A.np);
-        const size_t with_sc = base + sizeof(double) * (size_t)nodes * (nodes + 1);
-        A.sc_lds = with_sc <= 150 * 1024;
-        hipLaunchKernelGGL(rww_adj_pipe_kernel<512>, dim3(n_sims), dim3(2 * A.ks * A.np), A.sc_lds ? with_sc : base,
-                           0, A, c, bw);
-        return finish(hipGetLastError());
-    }
     const size_t base = sizeof(double) * (2 * (size_t)nodes + 16 + (size_t)(A.ks - 1) * A.np);
     const size_t with_sc = base + sizeof(double) * (size_t)nodes * (nodes + 1);
     A.sc_lds = with_sc <= 150 * 1024;

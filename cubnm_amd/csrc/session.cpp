@@ -882,7 +882,6 @@ using namespace cubnm;
 struct TuneChoice { int ks = 0; int pair = 0; double trial_ms = 0; };
 std::map<std::string, TuneChoice> g_tune_cache;
 bool g_tune_loaded = false;
-extern int g_adjoint_pipe;  // adjoint.hip
 int g_autotune = 0;   // -1 never, 0 large launches, 1 always
 int g_group_graph = 1;  // capture and replay the merged launch
 
@@ -989,11 +988,6 @@ int cubnm_set_tuning(const char* key, int value) {
     if (std::strcmp(key, "fc_mode") == 0) {
         if (value < 0 || value > 2) { g_err = "fc_mode must be 0 (auto), 1 (FP64 Gram) or 2 (split-BF16 Gram)"; return -1; }
         g_fc_mode = value;
-        return 0;
-    }
-    if (std::strcmp(key, "adjoint_pipe") == 0) {
-        if (value < 0 || value > 1) { g_err = "adjoint_pipe must be 0 (recompute, then sweep) or 1 (overlapped)"; return -1; }
-        g_adjoint_pipe = value;
         return 0;
     }
     g_err = std::string("unknown tuning key ") + key;

@@ -205,12 +205,22 @@ class RegionalFit:
             return self.loss.value(bold)
 
     def loss_and_grad(self, U):
+        loss, finish = self.loss_then_grad(U)
+        return loss, finish()
+
+    def loss_then_grad(self, U):
+        """The loss of each row of U (forward pass only) and a function that
+        runs the backward pass and returns the gradient, so that a caller
+        that stops on the loss skips the backward pass of its last point."""
         Uv = U.clone().requires_grad_(True)
         theta = self.lo + (self.hi - self.lo) * Uv
         G, wp, JN = self._split(theta)
         loss = self.loss(grad.rww_bold(self.p, G, wp, JN))
-        loss.sum().backward()
-        return loss.detach(), Uv.grad.detach()
+
+        def finish():
+            loss.sum().backward()
+            return Uv.grad.detach()
+        return loss.detach(), finish
 
 
 def cmaes_fit(fit, n_gen, seed=0, sigma=0.5, log=None):
@@ -239,8 +249,9 @@ def cmaes_fit(fit, n_gen, seed=0, sigma=0.5, log=None):
 
 
 def adam_fit(fit, n_steps, lr=0.02, seed=0, target=None, log=None):
-    """Adam from a uniform batch of starts; stops early at `target`; returns
-    the (wall seconds, best loss so far) history."""
+    """Adam from a uniform batch of starts; stops early at `target` (checked
+    on each point's loss before its gradient is computed); returns the (wall
+    seconds, best loss so far) history."""
     rng = np.random.default_rng(seed)
     dim = 1 + 2 * fit.N
     U = torch.tensor(rng.uniform(0, 1, (fit.p.n_sims, dim)), dtype=torch.float64, device=fit.p.device)
@@ -251,15 +262,16 @@ def adam_fit(fit, n_steps, lr=0.02, seed=0, target=None, log=None):
     hist = []
     best = float("inf")
     for k in range(1, n_steps + 1):
-        loss, g = fit.loss_and_grad(U)
+        loss, finish = fit.loss_then_grad(U)
         lf = loss.cpu().numpy()
         best = min(best, float(np.nanmin(np.where(np.isnan(lf), np.inf, lf))))
         hist.append((time.time() - t0, best))
         if log:
             log(f"adam step {k}: best {best:.5f} at {hist[-1][0]:.1f} s")
-        if target is not None and best <= target:
+        # the gradient is needed only to take another step
+        if (target is not None and best <= target) or k == n_steps:
             break
-        g = torch.nan_to_num(g)
+        g = torch.nan_to_num(finish())
         m = b1 * m + (1 - b1) * g
         v = b2 * v + (1 - b2) * g * g
         U = (U - lr * (m / (1 - b1 ** k)) / (torch.sqrt(v / (1 - b2 ** k)) + eps)).clamp(0.0, 1.0)

@@ -175,11 +175,7 @@ int cubnm_last_integrator(void);
  *                   simulation's BOLD fits in LDS, else FP64 Gram products),
  *                   1 = FP64 Gram products on MFMA, 2 = split-BF16 Gram
  *                   products (3 BF16 MFMA terms, FP32 accumulation flushed to
- *                   FP64; normalisations in FP64);
- *   "adjoint_pipe" = backward pass of cubnm_rww_grad_backward up to 256
- *                   nodes: 1 recomputes segment k-1 in a second group of
- *                   waves while the adjoint runs through segment k, 0
- *                   (default) recomputes, then sweeps, one segment at a time. */
+ *                   FP64; normalisations in FP64). */
 int cubnm_set_tuning(const char* key, int value);
 /* FC/FCD path of the last run: 0 LDS kernel, 1 FP64 Gram, 2 split-BF16 Gram. */
 int cubnm_last_fc_path(void);
@@ -200,9 +196,7 @@ int cubnm_copy_noise(double* noise, int* shuffled_nodes, int* shuffled_ts);
  * w_p, J_N, wIE, sigma [n_sims][nodes]; BOLD and dL/dBOLD
  * [n_sims][bold_len][nodes].  Timing arguments as cubnm_prepare.  The
  * forward pass stores a checkpoint every `ck` BW steps; the backward pass
- * recomputes one segment at a time (up to 256 nodes, the recomputation of
- * the next segment overlaps the adjoint sweep of the current one; scratch
- * then holds two segments).  Buffer sizes (doubles) come from
+ * recomputes one segment at a time.  Buffer sizes (doubles) come from
  * cubnm_rww_grad_sizes.  No FIC, no delay (wIE is an input). */
 int cubnm_rww_grad_sizes(int n_sims, int nodes, int time_steps, int BOLD_TR, double dt, double bw_dt, int ck,
                          long long* bold_doubles, long long* ckpt_doubles, long long* scratch_doubles);
