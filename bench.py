@@ -250,6 +250,8 @@ def main():
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = B * N * dur / (ms_per_step / 1000.0)
 
+    shape = capi.last_launch_shape()
+    log(f"launch shape: {shape}")
     corr, ks = d_corr.cpu().numpy(), d_ks.cpu().numpy()
     scored = min(S, counter[0] * B)
     log(f"scores of the first {scored} simulations: fc_corr mean {np.nanmean(corr[:scored]):.4f}, "
@@ -261,7 +263,8 @@ def main():
     traffic = read_traffic(os.path.join(ROOT, "profiles", "integrate_batched_rww_n100_c2_traffic.txt"))
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-                "kernel": "integrate_batched_kernel", "kernel_ms": round(k_ms, 2),
+                "kernel": ("integrate_batched_kernel" if shape["integrator"] == 1 else "integrate_general_kernel"),
+                "kernel_ms": round(k_ms, 2),
                 "fc_fcd_ms": round(float(np.mean(fc_ms)), 2)}
 
     cpu = None
@@ -280,7 +283,10 @@ def main():
                                f"FCD window 10 step 2, FIC off), FC + FCD + FC-Pearson + FCD-KS on GPU, "
                                f"{B} simulations per step ({n_batches} steps per pass over the grid)",
                    "sims": S, "sims_per_step": B, "nodes": N, "duration_s": dur,
-                   "integrator": "batched MFMA FP64"},
+                   "integrator": ["per-simulation", "batched MFMA FP64", "many-node"][shape["integrator"]],
+                   "launch_shape": {"tiles_per_wave": shape["tiles_per_wave"], "sc_in_lds": shape["sc_in_lds"],
+                                    "ksplit": shape["ksplit"],
+                                    "chosen_by": ["rule", "autotuner trials", "autotuner cache"][shape["chosen_by"]]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -46,6 +46,7 @@ def lib():
     L.cubnm_last_timings.argtypes = [ctypes.POINTER(ctypes.c_double * 6)]
     L.cubnm_force_integrator.argtypes = [I]
     L.cubnm_last_integrator.restype = I
+    L.cubnm_last_launch_shape.argtypes = [ctypes.POINTER(ctypes.c_int * 5)]
     L.cubnm_set_tuning.argtypes = [C, I]
     L.cubnm_gof_fc_corr.argtypes = [P, I, I, P, P]
     L.cubnm_gof_fcd_ks.argtypes = [P, I, I, P, I, P]
@@ -99,6 +100,15 @@ def last_timings():
 
 def set_tuning(key, value):
     _check(lib().cubnm_set_tuning(key.encode(), int(value)))
+
+
+def last_launch_shape():
+    """Launch shape of the last run: integrator, tiles per wave, SC fragments
+    in LDS, k partitions, chosen by (0 rule, 1 autotuner trials, 2 cache)."""
+    o = (ctypes.c_int * 5)()
+    lib().cubnm_last_launch_shape(ctypes.byref(o))
+    keys = ["integrator", "tiles_per_wave", "sc_in_lds", "ksplit", "chosen_by"]
+    return dict(zip(keys, list(o)))
 
 
 def force_integrator(k):

@@ -307,12 +307,14 @@ hipError_t launch_one(const BatchedArgs& ba, const void* consts, int waves, hipS
     return hipGetLastError();
 }
 
+// alds: SC fragments in LDS when they fit (1, or -1 = automatic) or
+// streamed from L2 (0).
 template <class M>
-hipError_t dispatch(const BatchedArgs& ba, const void* consts, int tpw, hipStream_t st) {
+hipError_t dispatch(const BatchedArgs& ba, const void* consts, int tpw, int alds, hipStream_t st) {
     const int waves = ba.NT / tpw;  // NT is padded to a multiple of tpw
     const int nx = M::is_osc ? 2 : 1;
     const size_t lds_a = sizeof(double) * ((size_t)ba.NT * ba.KT * 64 + 2 * nx * 4 * ba.KT * 16);
-    const bool a_lds = lds_a <= 150 * 1024;
+    const bool a_lds = alds != 0 && lds_a <= 150 * 1024;
     if (tpw == 1) {
         if (waves <= 4) return a_lds ? launch_one<M, 1, 4, true>(ba, consts, waves, st) : launch_one<M, 1, 4, false>(ba, consts, waves, st);
         if (waves <= 8 && a_lds && ba.KT == 26) return launch_one<M, 1, 8, true, 26>(ba, consts, waves, st);  // N = 101..104 and 97..100
@@ -339,6 +341,18 @@ hipError_t launch_integrate_batched_wide(int model, const IntegrateArgs& a, cons
 // dynamic states per node (integrate_batched_wide.hip).
 bool batched_supported(int model, int nodes) {
     return (nodes >= 1 && nodes <= 256) || batched_wide_supported(model, nodes);
+}
+
+void batched_plan(int nodes, int want_tpw, int* tpw, int* nt_pad);
+
+// Whether the SC fragments of a launch fit in LDS beside the coupling-state
+// buffers (what dispatch does with alds = -1).
+bool batched_afrag_fits_lds(int model, int nodes, int tpw) {
+    if (nodes > 256) return false;
+    int t, nt_pad;
+    batched_plan(nodes, tpw, &t, &nt_pad);
+    const int KT = batched_kt(nodes), nx = model == MODEL_KURAMOTO ? 2 : 1;
+    return sizeof(double) * ((size_t)nt_pad * KT * 64 + 2 * nx * 4 * KT * 16) <= 150 * 1024;
 }
 
 // Tiles per wave and the padded tile count (waves * TPW) the fragment array
@@ -381,7 +395,7 @@ hipError_t build_afrag(const double* d_SC, int n_scs, int nodes, int nt_pad, dou
 
 hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
                                     int* launches, const int* d_group_sims, int n_groups, const double* d_afrag,
-                                    int want_tpw, double* d_lanes) {
+                                    int want_tpw, int alds, double* d_lanes) {
     if (a.nodes > 256) {
         *launches = 1;
         return launch_integrate_batched_wide(model, a, consts, s, d_group_sims, n_groups, d_afrag, batched_kt(a.nodes),
@@ -398,11 +412,11 @@ hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const voi
     ba.NT = nt_pad;
     *launches = 1;
     switch (model) {
-        case MODEL_RWW: return dispatch<RWW>(ba, consts, tpw, s);
-        case MODEL_JR: return dispatch<JR>(ba, consts, tpw, s);
-        case MODEL_KURAMOTO: return dispatch<Kuramoto>(ba, consts, tpw, s);
-        case MODEL_WC: return dispatch<WC>(ba, consts, tpw, s);
-        case MODEL_RWWEX: return dispatch<RWWEx>(ba, consts, tpw, s);
+        case MODEL_RWW: return dispatch<RWW>(ba, consts, tpw, alds, s);
+        case MODEL_JR: return dispatch<JR>(ba, consts, tpw, alds, s);
+        case MODEL_KURAMOTO: return dispatch<Kuramoto>(ba, consts, tpw, alds, s);
+        case MODEL_WC: return dispatch<WC>(ba, consts, tpw, alds, s);
+        case MODEL_RWWEX: return dispatch<RWWEx>(ba, consts, tpw, alds, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -38,7 +38,8 @@ namespace cubnm {
 
 hipError_t launch_integrate_batched(int model, const IntegrateArgs& a, const void* consts, hipStream_t s,
                                     int* launches, const int* d_group_sims, int n_groups, const double* d_afrag,
-                                    int want_tpw, double* d_lanes);
+                                    int want_tpw, int alds, double* d_lanes);
+bool batched_afrag_fits_lds(int model, int nodes, int tpw);
 bool batched_supported(int model, int nodes);
 int batched_wide_grid(int n_groups);
 size_t batched_wide_lane_doubles(int model, int nodes, int grid);
@@ -170,6 +171,9 @@ struct Session {
     int shuffled_repeats = -1;
     std::vector<int> sh_nodes, sh_ts;
     int tpw = 0;              // batched tiles per wave for this session (0 = global knob)
+    int tune_tpw = 0;         // batched tiles per wave chosen by the autotuner for the last request (0 = none)
+    int alds = -1;            // batched SC fragments: -1 LDS when they fit, 0 streamed from L2
+    int shape[5] = {-1, 0, 0, 0, 0};  // last launch: integrator, tiles per wave, fragments in LDS, k partitions, chosen by
     // device buffers
     DevBuf d_shn, d_sht, d_pairs, d_wins, d_wpairs;
     DevBuf d_SC, d_SCi, d_gp, d_rp, d_v, d_bold, d_fc, d_fcd, d_states, d_gb, d_gi, d_delays, d_maxd, d_hist, d_wfc;
@@ -555,6 +559,141 @@ int build_groups(Session& s, const int* SCi_host) {
 // Integrator arguments for the session's request; uploads delays, decides
 // batched vs per-simulation integrator and builds the batched groups and SC
 // fragments (queued on the session's stream).
+// Launch-shape cache shared by the autotuners of single requests and of the
+// merged per-simulation launch: key -> choice, in memory and in a text file
+// ($CUBNM_AMD_TUNE_CACHE, else ~/.cache/cubnm_amd/tune.txt) so later
+// processes on the same machine reuse it.  The two integers of a choice
+// depend on the key's kind (see tune_single and cubnm_run_groups).
+struct TuneChoice { int ks = 0; int pair = 0; double trial_ms = 0; };
+std::map<std::string, TuneChoice> g_tune_cache;
+bool g_tune_loaded = false;
+int g_autotune = 0;   // -1 never, 0 large launches, 1 always
+int g_group_graph = 1;  // capture and replay the merged launch
+
+std::string tune_cache_path() {
+    if (const char* p = std::getenv("CUBNM_AMD_TUNE_CACHE")) return p;
+    const char* h = std::getenv("HOME");
+    return std::string(h ? h : "/tmp") + "/.cache/cubnm_amd/tune.txt";
+}
+
+void tune_cache_load() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    std::ifstream f(tune_cache_path());
+    std::string key;
+    TuneChoice c;
+    while (f >> key >> c.ks >> c.pair >> c.trial_ms) g_tune_cache[key] = c;
+}
+
+void tune_cache_store(const std::string& key, const TuneChoice& c) {
+    g_tune_cache[key] = c;
+    try {
+        const std::string path = tune_cache_path();
+        const size_t slash = path.rfind('/');
+        if (slash != std::string::npos) {
+            std::string dir = path.substr(0, slash), cur;
+            for (size_t i = 1; i <= dir.size(); i++)
+                if (i == dir.size() || dir[i] == '/') { cur = dir.substr(0, i); mkdir(cur.c_str(), 0755); }
+        }
+        std::ofstream f(path, std::ios::app);
+        f << key << " " << c.ks << " " << c.pair << " " << c.trial_ms << "\n";
+    } catch (...) {
+    }
+}
+
+
+// Autotuner of a single request that the batched integrator can run
+// (no delay, no numerical-FIC restarts, up to 256 nodes): times, on the
+// request's own launch cut to 50 BW steps (the real launch overwrites every
+// output), each candidate of
+//   simulations per workgroup: 16 (batched) or 1 (per-simulation);
+//   nodes per lane (batched): 4 or 8 (one or two 16-node tiles per wave);
+//   LDS layout (batched): SC fragments in LDS (when they fit) or streamed
+//     from L2; (per-simulation) 1, 2 or 4 k partitions of the coupling sum,
+//     SC staging and pair mode by the integrator's rule;
+// and keeps the fastest, cached per (model, nodes, simulations rounded to a
+// power of two, simulations per SC rounded down to a power of two).  Cache
+// entry: ks = tiles per wave (0: per-simulation integrator), pair = SC
+// fragments in LDS (batched) or k partitions (per-simulation).  Runs for
+// launches of at least 2e10 node-steps (autotune 0) or always (autotune 1);
+// the batched_tpw, general_ksplit and integrator knobs switch it off.
+// Returns 1 when trials ran, 2 for a cached choice; sets the session's tiles
+// per wave and fragment layout, or switches it to the per-simulation
+// integrator with the chosen k partitions in `a`.
+int tune_single(Session& s, IntegrateArgs& a, const DevIO& io) {
+    const int N = s.nodes, mid = s.model->id;
+    tune_cache_load();
+    int sims_p2 = 1;
+    while (sims_p2 < s.N_SIMS) sims_p2 <<= 1;
+    int share = 1;
+    while (share * 2 <= s.N_SIMS / s.N_SCs) share <<= 1;
+    const std::string key = "single," + std::to_string(mid) + "|N" + std::to_string(N) + "|S" +
+                            std::to_string(sims_p2) + "|per_sc" + std::to_string(share);
+    TuneChoice best;
+    int by = 2;
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end() && it->second.ks >= 0 && it->second.ks <= 2) {
+        best = it->second;
+    } else {
+        by = 1;
+        IntegrateArgs t = a;
+        t.bw_it = std::min(a.bw_it, 50);
+        int tp, nt_max;
+        batched_plan(N, 2, &tp, &nt_max);
+        s.d_afrag.ensure(sizeof(double) * batched_afrag_doubles(N, nt_max, s.N_SCs));
+        hipEvent_t e0, e1;
+        check(hipEventCreate(&e0), "event");
+        check(hipEventCreate(&e1), "event");
+        best.trial_ms = 1e300;
+        auto timed = [&](const char* what, auto&& launch) {
+            check(hipEventRecord(e0, s.stream), "event");
+            check(launch(), what);
+            check(hipEventRecord(e1, s.stream), "event");
+            check(hipEventSynchronize(e1), "autotune trial");
+            return ev_ms(e0, e1);
+        };
+        for (int tpw : {1, 2}) {
+            int tt, nt_pad;
+            batched_plan(N, tpw, &tt, &nt_pad);
+            check(build_afrag(io.SC, s.N_SCs, N, nt_pad, s.d_afrag.as<double>(), s.stream), "build_afrag launch");
+            for (int alds : {1, 0}) {
+                if (alds && !batched_afrag_fits_lds(mid, N, tpw)) continue;
+                int launches = 1;
+                const double ms = timed("autotune batched trial", [&] {
+                    return launch_integrate_batched(mid, t, model_consts(s), s.stream, &launches, s.d_groups.as<int>(),
+                                                    s.n_groups, s.d_afrag.as<double>(), tpw, alds,
+                                                    s.d_lanes.as<double>());
+                });
+                if (ms < best.trial_ms) { best.trial_ms = ms; best.ks = tpw; best.pair = alds; }
+            }
+        }
+        for (int ks : {1, 2, 4}) {
+            if (ks * general_np(N) > 1024) continue;
+            t.ksplit = ks;
+            const double ms = timed("autotune per-simulation trial", [&] {
+                return launch_integrate_general(mid, t, model_consts(s), s.stream,
+                                                s.N_SIMS <= 512 || s.N_SIMS >= 4 * s.N_SCs);
+            });
+            if (ms < best.trial_ms) { best.trial_ms = ms; best.ks = 0; best.pair = ks; }
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        tune_cache_store(key, best);
+    }
+    if (best.ks == 0) {
+        s.last_integrator = 0;
+        a.ksplit = best.pair;
+    } else {
+        s.tune_tpw = best.ks;
+        s.alds = best.pair;
+    }
+    return by;
+}
+
+// Tiles per wave of the session's batched launches: the request's own
+// setting, else the autotuner's choice, else the global knob / rule.
+int eff_tpw(const Session& s) { return s.tpw != 0 ? s.tpw : s.tune_tpw; }
+
 IntegrateArgs setup_integrate(Session& s, const DevIO& io, const std::vector<int>* delays,
                               const std::vector<int>* maxd, const int* SCi_host) {
     const int N = s.nodes;
@@ -611,16 +750,37 @@ IntegrateArgs setup_integrate(Session& s, const DevIO& io, const std::vector<int
     }
     s.last_integrator = use_batched ? 1 : (N > 1024 ? 2 : 0);
     s.n_groups = 0;
+    s.tune_tpw = 0;
+    s.alds = -1;
+    int chosen_by = 0;
     if (use_batched) {
         s.n_groups = build_groups(s, SCi_host);
+        const long long work = (long long)s.N_SIMS * s.bw_it * s.inner_it * N;
+        if (N <= 256 && g_force_integrator < 0 && g_batched_tpw == 0 && s.tpw == 0 && g_general_ksplit == 0 &&
+            (g_autotune > 0 || (g_autotune == 0 && work >= 20000000000LL)))
+            chosen_by = tune_single(s, a, io);
+    }
+    if (s.last_integrator == 1) {
         int tpw, nt_pad;
-        batched_plan(N, s.tpw, &tpw, &nt_pad);
+        batched_plan(N, eff_tpw(s), &tpw, &nt_pad);
         s.d_afrag.ensure(sizeof(double) * batched_afrag_doubles(N, nt_pad, s.N_SCs));
         check(build_afrag(io.SC, s.N_SCs, N, nt_pad, s.d_afrag.as<double>(), s.stream), "build_afrag launch");
         if (N > 256)  // lane-ordered parameters and BW states of the wide integrator
             s.d_lanes.ensure(sizeof(double) *
                              batched_wide_lane_doubles(s.model->id, N, batched_wide_grid(s.n_groups)));
     }
+    // launch shape for cubnm_last_launch_shape
+    s.shape[0] = s.last_integrator;
+    s.shape[1] = 0; s.shape[2] = 0; s.shape[3] = 0;
+    if (s.last_integrator == 1 && N <= 256) {
+        int tpw, nt_pad;
+        batched_plan(N, eff_tpw(s), &tpw, &nt_pad);
+        s.shape[1] = tpw;
+        s.shape[2] = s.alds != 0 && batched_afrag_fits_lds(s.model->id, N, tpw);
+    } else if (s.last_integrator == 0) {
+        s.shape[3] = a.ksplit;
+    }
+    s.shape[4] = chosen_by;
     return a;
 }
 
@@ -631,7 +791,7 @@ void launch_integrator(Session& s, const IntegrateArgs& a) {
         check(launch_integrate_wide(s.model->id, a, model_consts(s), s.stream), "integrate_wide launch");
     else if (s.last_integrator == 1)
         check(launch_integrate_batched(s.model->id, a, model_consts(s), s.stream, &launches, s.d_groups.as<int>(),
-                                       s.n_groups, s.d_afrag.as<double>(), s.tpw, s.d_lanes.as<double>()),
+                                       s.n_groups, s.d_afrag.as<double>(), eff_tpw(s), s.alds, s.d_lanes.as<double>()),
               "integrate_batched launch");
     else check(launch_integrate_general(s.model->id, a, model_consts(s), s.stream,
                                         s.N_SIMS <= 512 || s.N_SIMS >= 4 * s.N_SCs),
@@ -879,43 +1039,6 @@ using namespace cubnm;
 // call's signature is identical (for example the generations of an
 // optimiser, whose parameters change but whose buffers do not).  The tables
 // live in pinned host memory owned by the cached graph.
-struct TuneChoice { int ks = 0; int pair = 0; double trial_ms = 0; };
-std::map<std::string, TuneChoice> g_tune_cache;
-bool g_tune_loaded = false;
-int g_autotune = 0;   // -1 never, 0 large launches, 1 always
-int g_group_graph = 1;  // capture and replay the merged launch
-
-std::string tune_cache_path() {
-    if (const char* p = std::getenv("CUBNM_AMD_TUNE_CACHE")) return p;
-    const char* h = std::getenv("HOME");
-    return std::string(h ? h : "/tmp") + "/.cache/cubnm_amd/tune.txt";
-}
-
-void tune_cache_load() {
-    if (g_tune_loaded) return;
-    g_tune_loaded = true;
-    std::ifstream f(tune_cache_path());
-    std::string key;
-    TuneChoice c;
-    while (f >> key >> c.ks >> c.pair >> c.trial_ms) g_tune_cache[key] = c;
-}
-
-void tune_cache_store(const std::string& key, const TuneChoice& c) {
-    g_tune_cache[key] = c;
-    try {
-        const std::string path = tune_cache_path();
-        const size_t slash = path.rfind('/');
-        if (slash != std::string::npos) {
-            std::string dir = path.substr(0, slash), cur;
-            for (size_t i = 1; i <= dir.size(); i++)
-                if (i == dir.size() || dir[i] == '/') { cur = dir.substr(0, i); mkdir(cur.c_str(), 0755); }
-        }
-        std::ofstream f(path, std::ios::app);
-        f << key << " " << c.ks << " " << c.pair << " " << c.trial_ms << "\n";
-    } catch (...) {
-    }
-}
-
 struct GroupGraph {
     std::vector<char> sig;
     hipGraphExec_t exec = nullptr;
@@ -1008,6 +1131,10 @@ int cubnm_copy_noise(double* noise, int* shuffled_nodes, int* shuffled_ts) {
 }
 
 int cubnm_last_integrator(void) { return g_s.last_integrator; }
+int cubnm_last_launch_shape(int* out) {
+    for (int i = 0; i < 5; i++) out[i] = g_s.shape[i];
+    return 0;
+}
 int cubnm_last_fc_path(void) { return g_s.last_fc_path; }
 
 int cubnm_last_timings(double* out6) {

@@ -151,6 +151,16 @@ int cubnm_last_timings(double* out6);
 int cubnm_force_integrator(int force);
 int cubnm_last_integrator(void);
 
+/* Launch shape of the last run (no counterpart in the reference): out5[0]
+ * integrator (0 per-simulation, 1 batched, 2 many-node), [1] 16-node tiles
+ * per wave of the batched integrator (up to 256 nodes), [2] its SC fragments
+ * in LDS (1) or streamed from L2 (0), [3] k partitions of the per-simulation
+ * integrator, [4] chosen by the rule (0), autotuner trials (1) or the
+ * autotuner's cache (2).  The autotuner (knob "autotune") times the
+ * batched shapes and the per-simulation integrator's k partitions on the
+ * request's own launch cut to 50 BW steps. */
+int cubnm_last_launch_shape(int* out5);
+
 /* Launch-shape and arithmetic knobs (no counterpart in the reference):
  *   "batched_tpw" = 16-node tiles per wave of the batched integrator
  *                   (0 = auto, 1, 2);
@@ -163,10 +173,14 @@ int cubnm_last_integrator(void);
  *                   latency: SCs shared or few simulations), 1 = whenever
  *                   eligible (even N >= 96, SC in global memory, no delay),
  *                   -1 = never;
- *   "autotune"    = merged per-simulation launch of cubnm_run_groups: time
- *                   short trials of its k partitions and pair mode and keep
- *                   the fastest, cached per launch kind in memory and in
+ *   "autotune"    = time short trials of the launch shapes and keep the
+ *                   fastest, cached per launch kind in memory and in
  *                   $CUBNM_AMD_TUNE_CACHE (else ~/.cache/cubnm_amd/tune.txt):
+ *                   for a single request the batched integrator can run,
+ *                   simulations per workgroup (16 batched or 1), tiles per
+ *                   wave and SC-fragment layout (batched), k partitions
+ *                   (per-simulation); for the merged per-simulation launch
+ *                   of cubnm_run_groups, its k partitions and pair mode;
  *                   -1 never, 0 = launches of >= 2e10 node-steps, 1 always;
  *   "group_graph" = capture that launch and its table uploads into a
  *                   hipGraph and replay it while the launch is unchanged

@@ -16,6 +16,7 @@
   default constants again after set_const restores them.
 * More than 1024 nodes: the many-node integrator against the oracle, and its
   refusal of conduction delays.
+* The single-request autotuner: trials, cache, outputs against the oracle.
 """
 import numpy as np
 import pytest
@@ -146,3 +147,44 @@ def test_many_nodes_rejects_delay():
                           sc_dist=cases.synthetic_sc(N, seed=3) * 50, v=1.0)
     with pytest.raises(RuntimeError, match="more than 1024 nodes"):
         _core.run_simulations(*args)
+
+
+@pytest.mark.parametrize("model,nodes", [("rWW", 68), ("Kuramoto", 100)])
+def test_single_request_autotuner(tmp_path, monkeypatch, model, nodes):
+    """The autotuner of a single request times the batched shapes (tiles per
+    wave 1 and 2, SC fragments in LDS and streamed) and the per-simulation
+    integrator's k partitions on the request's own launch cut to 50 BW
+    steps, keeps the fastest and caches it: the first call runs trials
+    (chosen_by 1), the second reads the cache (chosen_by 2) and gives the
+    same outputs bit for bit; both agree with the oracle (Kuramoto over a
+    short horizon at 1e-7, as the other Kuramoto tests)."""
+    import cubnm_amd  # noqa: F401
+    from cubnm_amd import _core, capi
+    monkeypatch.setenv("CUBNM_AMD_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    S = 32
+    sc = cases.synthetic_sc(nodes, seed=nodes)
+    gp, rp = cases.params(model, S, nodes, G=np.linspace(0.2, 1.0, S), do_fic=False)
+    cfg = cases.model_config(model, TR=0.5, window_size=2, window_step=1, bold_remove_s=0, do_fcd=False,
+                             do_fic=False)
+    args = cases.sim_args(model, sc, gp, rp, cfg, S, duration_s=3 if model == "Kuramoto" else 6, TR_s=0.5)
+    capi.set_tuning("autotune", 1)
+    try:
+        first = _core.run_simulations(*cases.copy_args(args))
+        shape1 = capi.last_launch_shape()
+        second = _core.run_simulations(*cases.copy_args(args))
+        shape2 = capi.last_launch_shape()
+    finally:
+        capi.set_tuning("autotune", 0)
+    assert shape1["chosen_by"] == 1 and shape2["chosen_by"] == 2, (shape1, shape2)
+    assert {k: v for k, v in shape1.items() if k != "chosen_by"} == {k: v for k, v in shape2.items() if k != "chosen_by"}
+    if shape1["integrator"] == 1:
+        assert shape1["tiles_per_wave"] in (1, 2) and shape1["sc_in_lds"] in (0, 1)
+    else:
+        assert shape1["integrator"] == 0 and shape1["ksplit"] in (1, 2, 4)
+    assert "single" in (tmp_path / "tune.txt").read_text()
+    for k in ["sim_bold", "sim_fc_trils", "_sim_states"]:
+        np.testing.assert_array_equal(first[k], second[k])
+    ref = cases.run_oracle(cases.copy_args(args))
+    tol = 1e-7 if model == "Kuramoto" else TOL
+    assert cases.bold_rel_err(first["sim_bold"], ref["sim_bold"]).max() <= tol
+    assert cases.max_abs_err(first["sim_fc_trils"], ref["sim_fc_trils"]) <= tol
