@@ -29,7 +29,7 @@ def _checkout(dest):
                            capture_output=True).stdout.decode().split("\0")
     n = 0
     for f in files:
-        if not f or f.startswith("_work/") or not os.path.isfile(os.path.join(ROOT, f)):
+        if not f or not os.path.isfile(os.path.join(ROOT, f)):
             continue
         os.makedirs(os.path.join(dest, os.path.dirname(f)), exist_ok=True)
         shutil.copy2(os.path.join(ROOT, f), os.path.join(dest, f))
