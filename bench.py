@@ -141,6 +141,8 @@ def main():
                     help="threads for the CPU baseline (0: every core the process may use)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tpw", type=int, default=0, help="batched integrator tiles per wave (0: auto)")
+    ap.add_argument("--autotune", type=int, default=0, choices=[-1, 0, 1],
+                    help="launch-shape autotuner: -1 never, 0 large launches (default), 1 always")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (PMC byte-counter calibration)")
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
@@ -188,6 +190,7 @@ def main():
     emp_fcd = np.sort(utils.calculate_fcd(emp, 10, 2))
 
     capi.set_tuning("batched_tpw", args.tpw)
+    capi.set_tuning("autotune", args.autotune)
     capi.force_integrator(-1)
     d = capi.prepare("rWW", cfg, B, N, int(dur * 1000), int(TR_s * 1000), int(TR_s * 1000), 0, dt, bw_dt)
     t = lambda a, dtype=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)

@@ -107,6 +107,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
 #pragma unroll
             for (int i = 0; i < M::n_state; i++) s[sl][i] = 0.0;
             M::init(mc, s[sl], rp[sl]);
+            M::scale_rp(mc, rp[sl]);  // the split steps' form of the parameters
             bx[sl] = 0.0; bf[sl] = 1.0; bn[sl] = 1.0; bq[sl] = 1.0;
             sn[sl] = 0.0; cs[sl] = 1.0;
             shj[sl] = a.sh_nodes[jr];

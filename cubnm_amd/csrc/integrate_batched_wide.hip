@@ -141,13 +141,13 @@ __global__ void __launch_bounds__(64 * WB_WAVES) integrate_batched_wide_kernel(W
                 double* ps = fresh(lw) + (4 * t + r) * NV * 64;
                 double rp[M::n_regional];
 #pragma unroll
-                for (int i = 0; i < M::n_regional; i++) {
-                    rp[i] = a.rp[(long long)i * NSN + (long long)simr * N + jr];
-                    ps[i * 64] = rp[i];
-                }
+                for (int i = 0; i < M::n_regional; i++) rp[i] = a.rp[(long long)i * NSN + (long long)simr * N + jr];
 #pragma unroll
                 for (int i = 0; i < M::n_state; i++) s[t][r][i] = 0.0;
                 M::init(mc, s[t][r], rp);
+                M::scale_rp(mc, rp);  // the split steps' form of the parameters
+#pragma unroll
+                for (int i = 0; i < M::n_regional; i++) ps[i * 64] = rp[i];
                 ps[(M::n_regional + 0) * 64] = 0.0;  // x
                 ps[(M::n_regional + 1) * 64] = 1.0;  // f
                 ps[(M::n_regional + 2) * 64] = 1.0;  // nu

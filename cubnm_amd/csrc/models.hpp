@@ -132,8 +132,12 @@ struct RWW {
     // cores and the noise loads are in flight; `post` finishes I_E, the
     // excitatory population and S_I.  The operations are those of `step`;
     // dS_I adds its noise term last (step adds it first), a difference in
-    // rounding only.
+    // rounding only.  The split steps take the noise scale sigma*sqrt(dt) in
+    // rp[3] (scale_rp, applied once when the parameters are loaded): one
+    // multiplication per noise term instead of two, (sigma sqrt(dt)) n
+    // instead of (n sigma) sqrt(dt), a difference in rounding only.
     static constexpr int n_tmp = 3;
+    CUBNM_HD static void scale_rp(const C& c, double* rp) { rp[3] *= c.sqrt_dt; }
     CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double n1,
                              double* t) {
         (void)gp; (void)n0; (void)n1;
@@ -149,10 +153,10 @@ struct RWW {
         s[0] = t[0] + g * gp[0] * rp[1] - t[1];
         const double aE = c.a_E * s[0] - c.b_E;
         s[2] = DIV(aE, 1 - EXP(-c.d_E * aE));
-        const double dSE = n0 * rp[3] * c.sqrt_dt + c.dt_gamma_E * ((1 - s[4]) * s[2]) - c.dt_itau_E * s[4];
+        const double dSE = n0 * rp[3] + c.dt_gamma_E * ((1 - s[4]) * s[2]) - c.dt_itau_E * s[4];
         s[4] += dSE;
         s[4] = fmax(0.0, fmin(1.0, s[4]));
-        s[5] += n1 * rp[3] * c.sqrt_dt + t[2];
+        s[5] += n1 * rp[3] + t[2];
         s[5] = fmax(0.0, fmin(1.0, s[5]));
     }
 };
@@ -191,6 +195,7 @@ struct JR {
     }
     // `step` split around the coupling input (see RWW::pre): only y4 reads g.
     static constexpr int n_tmp = 2;
+    CUBNM_HD static void scale_rp(const C&, double*) {}
     CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double,
                              double* t) {
         (void)gp;
@@ -234,6 +239,7 @@ struct Kuramoto {
     }
     // the whole step reads the coupling input
     static constexpr int n_tmp = 1;
+    CUBNM_HD static void scale_rp(const C&, double*) {}
     CUBNM_HD static void pre(const C&, double*, const double*, const double*, double, double, double*) {}
     CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
                               double n1, const double*) {
@@ -267,6 +273,7 @@ struct WC {
     // `step` split around the coupling input (see RWW::pre): the inhibitory
     // population does not read g.
     static constexpr int n_tmp = 1;
+    CUBNM_HD static void scale_rp(const C&, double*) {}
     CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double n1,
                              double* t) {
         (void)gp; (void)n0;
@@ -312,6 +319,7 @@ struct RWWEx {
     // `step` split around the coupling input (see RWW::pre): only w J_N S
     // precedes it.
     static constexpr int n_tmp = 1;
+    CUBNM_HD static void scale_rp(const C&, double*) {}
     CUBNM_HD static void pre(const C& c, double* s, const double*, const double* rp, double, double, double* t) {
         t[0] = rp[0] * c.J_N * s[2];
     }
@@ -342,14 +350,29 @@ CUBNM_HD void bw_step(const BWConsts& b, double s, double& x, double& f, double&
 // difference stays at rounding level) and costs a third of the device pow.
 // Special values follow pow: log(0) = -inf -> 0, log(x<0) = NaN -> NaN.
 // nu^(1/alpha) with the reference's alpha = 0.32 (bw.cpp:4-22; no entry point
-// changes it, and 1/alpha = 3.125 exactly in FP64): nu^3 * nu^(1/8) with
-// three square roots, a few ulp from pow.  There is no general-alpha path:
-// its log/exp coefficients, kept in registers across the integrators' loops,
+// changes it, and 1/alpha = 3.125 exactly in FP64): nu^3 * nu^(1/8).  The
+// eighth root comes from y = nu^(-1/8): three hardware reciprocal square
+// roots give it to about 2^-22 (each is good to 2^-23), two Newton steps
+// y <- y (1 + (1 - nu y^8)/8) (quadratic convergence) to rounding, and
+// nu^(1/8) = 1/y by the refined reciprocal: within 4 ulp of pow(nu, 3.125)
+// for nu in [2e-9, 5e8] (checked on 2e7 points with a starting error of up to
+// 1e-6), 23 instructions instead of the 63 of three IEEE square roots.  nu <= 0 gives NaN (pow gives 0 at nu = 0, where the BW q update
+// divides 0 by 0 one half-step later).  There is no general-alpha path: its
+// log/exp coefficients, kept in registers across the integrators' loops,
 // cost more registers than the rest of the BW step.
 __device__ inline double bw_nu_pow(const BWConsts& b, double nu) {
     (void)b;
-    const double r8 = sqrt(sqrt(sqrt(nu)));
-    return (nu * nu) * nu * r8;
+    double y = __builtin_amdgcn_rsq(__builtin_amdgcn_rsq(__builtin_amdgcn_rsq(nu)));
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        const double y2 = y * y, y4 = y2 * y2, y8 = y4 * y4;
+        const double e = fma(-nu, y8, 1.0);
+        y = fma(y, e * 0.125, y);
+    }
+    double r = __builtin_amdgcn_rcp(y);
+    r = fma(fma(-y, r, 1.0), r, r);
+    r = fma(fma(-y, r, 1.0), r, r);
+    return (nu * nu) * nu * r;
 }
 
 // The BW step of bw_step_explog with fewer instructions: powers of nu as

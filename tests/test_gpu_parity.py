@@ -157,10 +157,11 @@ def test_c1_plain_grid_rww(core):
     _compare(out, ref, "rWW")
 
 
-@pytest.mark.parametrize("nodes", [1, 5, 16, 33, 68, 130])
+@pytest.mark.parametrize("nodes", [1, 5, 16, 33, 68, 97, 104, 130])
 def test_batched_node_counts(core, nodes):
     """Tile padding: node counts that are not multiples of 4 or 16, one tile,
-    several tiles, and the L2-streamed SC fragments (N > 112)."""
+    several tiles, 97..104 nodes (7 tiles of 26 fragments: the compile-time
+    fragment count) and the L2-streamed SC fragments (N > 112)."""
     S = 19  # one full group of 16 plus a padded group
     sc = cases.synthetic_sc(nodes, seed=nodes)
     gp, rp = cases.params("rWW", S, nodes, G=np.linspace(0.2, 2.0, S), do_fic=False, wIE=1.0)
