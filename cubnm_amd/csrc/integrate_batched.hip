@@ -26,6 +26,7 @@
 // group (one seed), so its loads are broadcasts.  BOLD, states and FIC
 // outputs use the reference layouts.
 #include "engine.hpp"
+#include "exp_table.hpp"
 
 namespace cubnm {
 
@@ -58,8 +59,10 @@ __device__ __forceinline__ void publish(double* xs, int kn, int j, int c, double
 }  // namespace
 
 // A_LDS: A fragments staged in LDS (small N) or read from global memory.
-// KTC: compile-time fragment count per tile (0 = runtime ba.KT).
-template <class M, int TPW, int MAXW, bool A_LDS, int KTC>
+// KTC: compile-time fragment count per tile (0 = runtime ba.KT); KTU: the
+// fragments the product uses (ceil(N/4)), at most KTC (the layout keeps KTC
+// even; the last fragment is all zero when N <= 4 KTC - 4).
+template <class M, int TPW, int MAXW, bool A_LDS, int KTC, int KTU>
 __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArgs ba, typename M::C mc) {
     constexpr int NS = 4 * TPW;                 // node slots per lane
     constexpr int NX = M::is_osc ? 2 : 1;        // B operands per step
@@ -67,10 +70,16 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, q = lane >> 4;
     const int N = a.nodes, KT = KTC > 0 ? KTC : ba.KT, KN = 4 * KT, NT = ba.NT;
-    const int buf_stride = NX * KN * 16;        // doubles per X buffer
+    // a plane holds the KN source rows and one spare row (KN) that takes
+    // the writes of padding rows; no product reads it
+    const int KNP = KN + 1;
+    const int buf_stride = NX * KNP * 16;       // doubles per X buffer
     extern __shared__ double lds[];
-    double* xs = lds;                            // [2][NX][KN][16]
-    double* as = lds + 2 * buf_stride;           // [NT][KT][64] when A_LDS
+    double* etab = lds;                          // [kExpTableSize] for ExpTable
+    double* xs = lds + kExpTableSize;            // [2][NX][KNP][16]
+    double* as = xs + 2 * buf_stride;            // [NT][KT][64] when A_LDS
+    load_exp_table(etab);
+    const ExpTable ex{etab};
     int loaded_sc = -1;
 
     for (int g = blockIdx.x; g < ba.n_groups; g += gridDim.x) {
@@ -93,7 +102,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
 
         double s[NS][M::n_state], rp[NS][M::n_regional];
         double bx[NS], bf[NS], bn[NS], bq[NS], sn[NS], cs[NS];
-        int shj[NS];
+        unsigned shb[NS];  // byte offset of the node's noise in a step's row
         // zero both X buffers (padding rows stay finite: A is zero there)
         for (int i = threadIdx.x; i < 2 * buf_stride; i += blockDim.x) xs[i] = 0.0;
         __syncthreads();
@@ -110,8 +119,8 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
             M::scale_rp(mc, rp[sl]);  // the split steps' form of the parameters
             bx[sl] = 0.0; bf[sl] = 1.0; bn[sl] = 1.0; bq[sl] = 1.0;
             sn[sl] = 0.0; cs[sl] = 1.0;
-            shj[sl] = a.sh_nodes[jr];
-            if (j < N) publish<M>(xs, KN, j, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
+            shb[sl] = (unsigned)a.sh_nodes[jr] * (unsigned)(sizeof(double) * M::n_noise);
+            if (j < N) publish<M>(xs, KNP, j, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
             if (j < N && sim_ok)
 #pragma unroll
                 for (int i = 0; i < M::n_state; i++) a.states[(long long)i * a.N_SIMS * N + (long long)sim * N + j] = 0.0;
@@ -129,10 +138,13 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
             for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
                 // this step's noise: issued before the coupling product so the
                 // loads complete under the MFMAs
+                // (a wave-uniform row address plus a 32-bit lane offset: the
+                // loads take the row in scalar registers, no vector address math)
                 double n0[NS], n1[NS];
+                const char* nrow = reinterpret_cast<const char*>(nz + (long long)inner_i * N * M::n_noise);
 #pragma unroll
                 for (int sl = 0; sl < NS; sl++) {
-                    const double* p = nz + ((long long)inner_i * N + shj[sl]) * M::n_noise;
+                    const double* p = reinterpret_cast<const double*>(nrow + shb[sl]);
                     n0[sl] = p[0];
                     n1[sl] = (M::n_noise > 1) ? p[1] : 0.0;
                 }
@@ -141,7 +153,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                 // scheduler can issue it between the MFMAs
                 double tp[NS][M::n_tmp];
 #pragma unroll
-                for (int sl = 0; sl < NS; sl++) M::pre(mc, s[sl], gp, rp[sl], n0[sl], n1[sl], tp[sl]);
+                for (int sl = 0; sl < NS; sl++) M::pre(mc, s[sl], gp, rp[sl], n0[sl], n1[sl], tp[sl], ex);
                 const double* xb = xs + buf * buf_stride;
                 // two accumulators per product (even and odd source
                 // fragments), added at the end: two independent MFMA chains
@@ -159,10 +171,10 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                 if constexpr (KTC > 0) {
                     // compile-time fragment count: fully unrolled, immediate LDS offsets
 #pragma unroll
-                    for (int kt = 0; kt < KTC; kt++) {
+                    for (int kt = 0; kt < KTU; kt++) {
 #pragma unroll
                         for (int x = 0; x < NX; x++) {
-                            const double b = bb[x * KN * 16 + kt * 64];
+                            const double b = bb[x * KNP * 16 + kt * 64];
 #pragma unroll
                             for (int t = 0; t < TPW; t++) {
                                 if (kt & 1)
@@ -180,14 +192,14 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
 #pragma unroll
                     for (int t = 0; t < TPW; t++) { a0[t] = ab[t * KT * 64]; a1[t] = ab[(t * KT + 1) * 64]; }
 #pragma unroll
-                    for (int x = 0; x < NX; x++) { b0[x] = bb[x * KN * 16]; b1[x] = bb[x * KN * 16 + 64]; }
+                    for (int x = 0; x < NX; x++) { b0[x] = bb[x * KNP * 16]; b1[x] = bb[x * KNP * 16 + 64]; }
                     for (int kt = 0; kt < KT; kt += 2) {
                         const int k2 = min(kt + 2, KT - 2), k3 = k2 + 1;
                         double a2[TPW], a3[TPW], b2[NX], b3[NX];
 #pragma unroll
                         for (int t = 0; t < TPW; t++) { a2[t] = ab[(t * KT + k2) * 64]; a3[t] = ab[(t * KT + k3) * 64]; }
 #pragma unroll
-                        for (int x = 0; x < NX; x++) { b2[x] = bb[x * KN * 16 + k2 * 64]; b3[x] = bb[x * KN * 16 + k3 * 64]; }
+                        for (int x = 0; x < NX; x++) { b2[x] = bb[x * KNP * 16 + k2 * 64]; b3[x] = bb[x * KNP * 16 + k3 * 64]; }
                         // keep the prefetch ahead of the MFMAs consuming the previous pair
                         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -209,25 +221,34 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
 #pragma unroll
                     for (int x = 0; x < NX; x++) acc[t][x] += acc2[t][x];
                 double* xn = xs + (buf ^ 1) * buf_stride;
-                const bool last = sample && (inner_i == a.inner_it - 1);
 #pragma unroll
                 for (int t = 0; t < TPW; t++)
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int sl = 4 * t + r;
-                        const int j = 16 * (wave * TPW + t) + 4 * r + q;
                         double gin;
                         if constexpr (M::is_osc) gin = cs[sl] * acc[t][0][r] - sn[sl] * acc[t][1][r];
                         else gin = acc[t][0][r];
-                        M::post(mc, s[sl], gp, rp[sl], gin, n0[sl], n1[sl], tp[sl]);
-                        if (j < N) {
-                            publish<M>(xn, KN, j, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
-                            if (last)
-#pragma unroll
-                                for (int i = 0; i < M::n_state; i++)
-                                    a.states[(long long)i * a.N_SIMS * N + (long long)sim * N + j] += s[sl][i];
-                        }
+                        M::post(mc, s[sl], gp, rp[sl], gin, n0[sl], n1[sl], tp[sl], ex);
                     }
+                // every slot writes its coupling state, padding rows into the
+                // spare row: no branch between the slots' steps, so they form
+                // one block whose dependent chains the scheduler interleaves
+#pragma unroll
+                for (int sl = 0; sl < NS; sl++) {
+                    const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
+                    publish<M>(xn, KNP, j < N ? j : KN, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
+                }
+                if (sample && inner_i == a.inner_it - 1) {
+#pragma unroll
+                    for (int sl = 0; sl < NS; sl++) {
+                        const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
+                        if (j < N)
+#pragma unroll
+                            for (int i = 0; i < M::n_state; i++)
+                                a.states[(long long)i * a.N_SIMS * N + (long long)sim * N + j] += s[sl][i];
+                    }
+                }
                 buf ^= 1;
                 __syncthreads();
             }
@@ -248,7 +269,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
 #pragma unroll
                 for (int sl = 0; sl < NS; sl++) {
                     const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
-                    shj[sl] = a.sh_nodes[rep * N + (j < N ? j : 0)];
+                    shb[sl] = (unsigned)a.sh_nodes[rep * N + (j < N ? j : 0)] * (unsigned)(sizeof(double) * M::n_noise);
                 }
             }
             bw_i++;
@@ -286,15 +307,15 @@ __global__ void build_afrag_kernel(const double* SC, int n_scs, int N, int NT, i
     }
 }
 
-template <class M, int TPW, int MAXW, bool A_LDS, int KTC = 0>
+template <class M, int TPW, int MAXW, bool A_LDS, int KTC = 0, int KTU = KTC>
 hipError_t launch_one(const BatchedArgs& ba, const void* consts, int waves, hipStream_t st) {
     if (waves > MAXW) return hipErrorInvalidConfiguration;
     const typename M::C mc = *static_cast<const typename M::C*>(consts);
     const int nx = M::is_osc ? 2 : 1;
-    size_t lds = sizeof(double) * 2 * nx * 4 * ba.KT * 16;
+    size_t lds = sizeof(double) * (kExpTableSize + 2 * nx * (4 * ba.KT + 1) * 16);
     if (A_LDS) lds += sizeof(double) * (size_t)ba.NT * ba.KT * 64;
-    if (KTC > 0 && ba.KT != KTC) return hipErrorInvalidConfiguration;
-    auto kern = integrate_batched_kernel<M, TPW, MAXW, A_LDS, KTC>;
+    if (KTC > 0 && (ba.KT != KTC || 4 * KTU < ba.base.nodes)) return hipErrorInvalidConfiguration;
+    auto kern = integrate_batched_kernel<M, TPW, MAXW, A_LDS, KTC, KTU>;
     const int threads = 64 * waves;
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds);
@@ -314,11 +335,14 @@ template <class M>
 hipError_t dispatch(const BatchedArgs& ba, const void* consts, int tpw, int alds, hipStream_t st) {
     const int waves = ba.NT / tpw;  // NT is padded to a multiple of tpw
     const int nx = M::is_osc ? 2 : 1;
-    const size_t lds_a = sizeof(double) * ((size_t)ba.NT * ba.KT * 64 + 2 * nx * 4 * ba.KT * 16);
+    const size_t lds_a = sizeof(double) * ((size_t)ba.NT * ba.KT * 64 + kExpTableSize + 2 * nx * (4 * ba.KT + 1) * 16);
     const bool a_lds = alds != 0 && lds_a <= 150 * 1024;
     if (tpw == 1) {
         if (waves <= 4) return a_lds ? launch_one<M, 1, 4, true>(ba, consts, waves, st) : launch_one<M, 1, 4, false>(ba, consts, waves, st);
-        if (waves <= 8 && a_lds && ba.KT == 26) return launch_one<M, 1, 8, true, 26>(ba, consts, waves, st);  // N = 101..104 and 97..100
+        if (waves <= 8 && a_lds && ba.KT == 26) {
+            if (ba.base.nodes <= 100) return launch_one<M, 1, 8, true, 26, 25>(ba, consts, waves, st);  // N = 97..100
+            return launch_one<M, 1, 8, true, 26, 26>(ba, consts, waves, st);                            // N = 101..104
+        }
         if (waves <= 8) return a_lds ? launch_one<M, 1, 8, true>(ba, consts, waves, st) : launch_one<M, 1, 8, false>(ba, consts, waves, st);
         return hipErrorInvalidValue;
     }
@@ -353,7 +377,7 @@ bool batched_afrag_fits_lds(int model, int nodes, int tpw) {
     int t, nt_pad;
     batched_plan(nodes, tpw, &t, &nt_pad);
     const int KT = batched_kt(nodes), nx = model == MODEL_KURAMOTO ? 2 : 1;
-    return sizeof(double) * ((size_t)nt_pad * KT * 64 + 2 * nx * 4 * KT * 16) <= 150 * 1024;
+    return sizeof(double) * ((size_t)nt_pad * KT * 64 + kExpTableSize + 2 * nx * (4 * KT + 1) * 16) <= 150 * 1024;
 }
 
 // Tiles per wave and the padded tile count (waves * TPW) the fragment array

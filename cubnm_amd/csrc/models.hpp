@@ -80,6 +80,12 @@ __device__ inline double exp_cw(double x) {
     return __builtin_ldexp(p, (int)k);
 }
 
+// The exponential of the split model steps (pre/post) as a functor, so that
+// an integrator can pass its own; this one is EXP.
+struct ExpPoly {
+    CUBNM_HD double operator()(double x) const { return EXP(x); }
+};
+
 // rWW ---------------------------------------------------------------------
 struct RWW {
     static constexpr int n_state = 6, n_noise = 2, n_global = 1, n_regional = 4;
@@ -138,21 +144,23 @@ struct RWW {
     // instead of (n sigma) sqrt(dt), a difference in rounding only.
     static constexpr int n_tmp = 3;
     CUBNM_HD static void scale_rp(const C& c, double* rp) { rp[3] *= c.sqrt_dt; }
+    template <class E = ExpPoly>
     CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double n1,
-                             double* t) {
+                             double* t, const E& ex = E()) {
         (void)gp; (void)n0; (void)n1;
         t[0] = c.w_E__I_0 + rp[0] * rp[1] * s[4];
         t[1] = rp[2] * s[5];
         s[1] = c.w_I__I_0 + rp[1] * s[4] - c.w_II * s[5];
         const double aI = c.a_I * s[1] - c.b_I;
-        s[3] = DIV(aI, 1 - EXP(-c.d_I * aI));
+        s[3] = DIV(aI, 1 - ex(-c.d_I * aI));
         t[2] = c.dt_gamma_I * s[3] - c.dt_itau_I * s[5];
     }
+    template <class E = ExpPoly>
     CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
-                              double n1, const double* t) {
+                              double n1, const double* t, const E& ex = E()) {
         s[0] = t[0] + g * gp[0] * rp[1] - t[1];
         const double aE = c.a_E * s[0] - c.b_E;
-        s[2] = DIV(aE, 1 - EXP(-c.d_E * aE));
+        s[2] = DIV(aE, 1 - ex(-c.d_E * aE));
         const double dSE = n0 * rp[3] + c.dt_gamma_E * ((1 - s[4]) * s[2]) - c.dt_itau_E * s[4];
         s[4] += dSE;
         s[4] = fmax(0.0, fmin(1.0, s[4]));
@@ -196,12 +204,13 @@ struct JR {
     // `step` split around the coupling input (see RWW::pre): only y4 reads g.
     static constexpr int n_tmp = 2;
     CUBNM_HD static void scale_rp(const C&, double*) {}
+    template <class E = ExpPoly>
     CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double,
-                             double* t) {
+                             double* t, const E& ex = E()) {
         (void)gp;
-        const double sg0 = DIV(2.0 * c.nu_max, 1.0 + EXP(c.r * (c.v0 - (s[1] - s[2]))));
-        const double sg1 = DIV(2.0 * c.nu_max, 1.0 + EXP(c.r * (c.v0 - (rp[1] * rp[0] * s[0]))));
-        const double sg2 = DIV(2.0 * c.nu_max, 1.0 + EXP(c.r * (c.v0 - (rp[3] * rp[0] * s[0]))));
+        const double sg0 = DIV(2.0 * c.nu_max, 1.0 + ex(c.r * (c.v0 - (s[1] - s[2]))));
+        const double sg1 = DIV(2.0 * c.nu_max, 1.0 + ex(c.r * (c.v0 - (rp[1] * rp[0] * s[0]))));
+        const double sg2 = DIV(2.0 * c.nu_max, 1.0 + ex(c.r * (c.v0 - (rp[3] * rp[0] * s[0]))));
         const double d0 = s[3], d1 = s[4], d2 = s[5];
         const double d3 = c.A * c.a * sg0 - 2.0 * c.a * s[3] - c.a2 * s[0];
         t[0] = c.mu + rp[2] * rp[0] * sg1;
@@ -213,10 +222,11 @@ struct JR {
         s[3] += c.dt * d3 + c.sqrt_dt * n0 * rp[5];
         s[5] += c.dt * d5;
         s[6] = s[1] - s[2];
-        s[7] = c.cmin + DIV(c.cmax - c.cmin, 1.0 + EXP(c.r * (c.midpoint - (s[6]))));
+        s[7] = c.cmin + DIV(c.cmax - c.cmin, 1.0 + ex(c.r * (c.midpoint - (s[6]))));
     }
+    template <class E = ExpPoly>
     CUBNM_HD static void post(const C& c, double* s, const double* gp, const double*, double g, double, double,
-                              const double* t) {
+                              const double* t, const E& ex = E()) {
         const double d4 = c.A * c.a * (t[0] + gp[0] * g) - 2.0 * c.a * s[4] - t[1];
         s[4] += c.dt * d4;
     }
@@ -240,9 +250,11 @@ struct Kuramoto {
     // the whole step reads the coupling input
     static constexpr int n_tmp = 1;
     CUBNM_HD static void scale_rp(const C&, double*) {}
-    CUBNM_HD static void pre(const C&, double*, const double*, const double*, double, double, double*) {}
+    template <class E = ExpPoly>
+    CUBNM_HD static void pre(const C&, double*, const double*, const double*, double, double, double*, const E& = E()) {}
+    template <class E = ExpPoly>
     CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
-                              double n1, const double*) {
+                              double n1, const double*, const E& = E()) {
         step(c, s, gp, rp, g, n0, n1);
     }
 };
@@ -274,18 +286,20 @@ struct WC {
     // population does not read g.
     static constexpr int n_tmp = 1;
     CUBNM_HD static void scale_rp(const C&, double*) {}
+    template <class E = ExpPoly>
     CUBNM_HD static void pre(const C& c, double* s, const double* gp, const double* rp, double n0, double n1,
-                             double* t) {
+                             double* t, const E& ex = E()) {
         (void)gp; (void)n0;
         t[0] = rp[0] * s[0] - rp[1] * s[1];
-        const double SI = DIV(1.0, 1.0 + EXP(-c.a_I * (rp[2] * s[0] - rp[3] * s[1] + rp[5] - c.mu_I)));
+        const double SI = DIV(1.0, 1.0 + ex(-c.a_I * (rp[2] * s[0] - rp[3] * s[1] + rp[5] - c.mu_I)));
         const double dI = (-s[1] + (1.0 - s[1]) * SI) * c.itau_I;
         s[1] += c.dt * dI + c.sqrt_dt * n1 * rp[7];
         s[1] = fmax(0.0, fmin(1.0, s[1]));
     }
+    template <class E = ExpPoly>
     CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
-                              double, const double* t) {
-        const double SE = DIV(1.0, 1.0 + EXP(-c.a_E * (t[0] + gp[0] * g + rp[4] - c.mu_E)));
+                              double, const double* t, const E& ex = E()) {
+        const double SE = DIV(1.0, 1.0 + ex(-c.a_E * (t[0] + gp[0] * g + rp[4] - c.mu_E)));
         const double dE = (-s[0] + (1.0 - s[0]) * SE) * c.itau_E;
         s[0] += c.dt * dE + c.sqrt_dt * n0 * rp[6];
         s[0] = fmax(0.0, fmin(1.0, s[0]));
@@ -320,14 +334,16 @@ struct RWWEx {
     // precedes it.
     static constexpr int n_tmp = 1;
     CUBNM_HD static void scale_rp(const C&, double*) {}
-    CUBNM_HD static void pre(const C& c, double* s, const double*, const double* rp, double, double, double* t) {
+    template <class E = ExpPoly>
+    CUBNM_HD static void pre(const C& c, double* s, const double*, const double* rp, double, double, double* t, const E& ex = E()) {
         t[0] = rp[0] * c.J_N * s[2];
     }
+    template <class E = ExpPoly>
     CUBNM_HD static void post(const C& c, double* s, const double* gp, const double* rp, double g, double n0,
-                              double, const double* t) {
+                              double, const double* t, const E& ex = E()) {
         s[0] = t[0] + gp[0] * c.J_N * g + rp[1];
         const double axb = c.a * s[0] - c.b;
-        s[1] = DIV(axb, 1 - EXP(-c.d * axb));
+        s[1] = DIV(axb, 1 - ex(-c.d * axb));
         const double dS = c.dt_gamma * ((1 - s[2]) * s[1]) - c.dt_itau * s[2] + n0 * c.sqrt_dt * rp[2];
         s[2] += dS;
         s[2] = fmax(0.0, fmin(1.0, s[2]));
