@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                 __syncthreads();
             }
 #pragma unroll
-            for (int sl = 0; sl < NS; sl++) bw_step_fast(a.bw, s[sl][M::bold_idx], bx[sl], bf[sl], bn[sl], bq[sl]);
+            for (int sl = 0; sl < NS; sl++) bw_step_fast(a.bw, s[sl][M::bold_idx], bx[sl], bf[sl], bn[sl], bq[sl], ex);
             if ((bw_i + 1) % a.BOLD_TR_iters == 0) {
                 if (sim_ok && bold_i < a.bold_len)
 #pragma unroll

@@ -28,6 +28,7 @@
 //     so the whole model step runs after the product; the second wave on the
 //     SIMD fills the latency of the step's parameter and noise loads.
 #include "engine.hpp"
+#include "exp_table.hpp"
 
 namespace cubnm {
 
@@ -93,6 +94,9 @@ __global__ void __launch_bounds__(64 * WB_WAVES) integrate_batched_wide_kernel(W
     double* xs = lds;                            // [2][KN][16]
     double* nzs = lds + 2 * KN * 16;             // [2][row]
     double* trash = nzs + 2 * row;               // [64]
+    double* etab = trash + 64;                   // [kExpTableSize] for ExpTable
+    load_exp_table(etab);
+    const ExpTable ex{etab};
     // lane copy of this wave: [slot = 4t + r][NV][64 lanes]
     double* const lw = wa.lanes + (long long)(blockIdx.x * WB_WAVES + wave) * TM * 4 * NV * 64 + lane;
     // node of slot (t, r) is j0 + 128 t + 4 r
@@ -240,8 +244,8 @@ __global__ void __launch_bounds__(64 * WB_WAVES) integrate_batched_wide_kernel(W
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         double tp[M::n_tmp];
-                        M::pre(mc, s[t][r], gp, rp[r], n0[r], n1[r], tp);
-                        M::post(mc, s[t][r], gp, rp[r], acc[t][r], n0[r], n1[r], tp);
+                        M::pre(mc, s[t][r], gp, rp[r], n0[r], n1[r], tp, ex);
+                        M::post(mc, s[t][r], gp, rp[r], acc[t][r], n0[r], n1[r], tp, ex);
                         const bool valid = full || fresh_int(j0) + 128 * t + 4 * r < N;
                         *(valid ? xn + (128 * t + 4 * r) * 16 : trash + lane) = s[t][r][M::conn_idx];
                         if (last && valid && sim_ok) {
@@ -268,7 +272,7 @@ __global__ void __launch_bounds__(64 * WB_WAVES) integrate_batched_wide_kernel(W
                 for (int r = 0; r < 4; r++) {
                     double* ps = fresh(lw) + ((4 * t + r) * NV + M::n_regional) * 64;
                     double x = ps[0], f = ps[64], nu = ps[128], qq = ps[192];
-                    bw_step_fast(a.bw, s[t][r][M::bold_idx], x, f, nu, qq);
+                    bw_step_fast(a.bw, s[t][r][M::bold_idx], x, f, nu, qq, ex);
                     ps[0] = x; ps[64] = f; ps[128] = nu; ps[192] = qq;
                     if (out && (full || fresh_int(j0) + 128 * t + 4 * r < N))
                         fresh(bold_lane)[(long long)bold_i * N + 128 * t + 4 * r] = bold_signal(a.bw, nu, qq);
@@ -311,7 +315,7 @@ __global__ void __launch_bounds__(64 * WB_WAVES) integrate_batched_wide_kernel(W
 namespace {
 
 size_t batched_wide_lds_bytes(int KT, int nodes, int n_noise) {
-    return sizeof(double) * (2 * 4 * (size_t)KT * 16 + 2 * (size_t)nodes * n_noise + 64);
+    return sizeof(double) * (2 * 4 * (size_t)KT * 16 + 2 * (size_t)nodes * n_noise + 64 + kExpTableSize);
 }
 
 template <class M, int TMIN>

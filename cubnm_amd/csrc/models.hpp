@@ -392,13 +392,20 @@ __device__ inline double bw_nu_pow(const BWConsts& b, double nu) {
 }
 
 // The BW step of bw_step_explog with fewer instructions: powers of nu as
-// above, the divisions by f and nu through fast_div, 1/rho as a product.
-__device__ inline void bw_step_fast(const BWConsts& b, double s, double& x, double& f, double& nu, double& q) {
+// above, the divisions by f and nu through fast_div, 1/rho as a product.  The
+// exponential is a functor (the C library's exp unless the caller passes its
+// own).
+struct ExpLib {
+    __device__ double operator()(double x) const { return exp(x); }
+};
+template <class E = ExpLib>
+__device__ inline void bw_step_fast(const BWConsts& b, double s, double& x, double& f, double& nu, double& q,
+                                    const E& ex = E()) {
     x = x + b.dt * (s - b.kappa * x - b.y * (f - 1.0));
     const double tmp_f = f + b.dt * x;
     nu = nu + b.dt_itau * (f - bw_nu_pow(b, nu));
     const double nu_pow = bw_nu_pow(b, nu);
-    q = q + b.dt_itau * (f * (1.0 - exp(fast_div(b.log_oneminrho, f))) * b.irho - fast_div(nu_pow * q, nu));
+    q = q + b.dt_itau * (f * (1.0 - ex(fast_div(b.log_oneminrho, f))) * b.irho - fast_div(nu_pow * q, nu));
     f = tmp_f;
 }
 
