@@ -31,6 +31,12 @@
 namespace cubnm {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
+// no swizzle); out-of-range loads return zero
+constexpr int kBufferConfig = 0x00020000;
 
 struct BatchedArgs {
     IntegrateArgs base;
@@ -130,23 +136,34 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
         double* bold = a.bold + (long long)simr * a.bold_len * N;
         int buf = 0, bw_i = 0, bold_i = 0, states_i = 0, rep = 0;
         while (bw_i < a.bw_it) {
-            const int sh_t = a.sh_ts[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it];
+            // (readfirstlane: the value is the same in every lane; it keeps the
+            // noise buffer below in scalar registers)
+            const int sh_t = __builtin_amdgcn_readfirstlane(a.sh_ts[(bw_i % a.noise_bw_it) + rep * a.noise_bw_it]);
             // states are summed from the last inner step of a sampling bw step
             const bool sample = ((bw_i + 1) % a.states_sampling_iters == 0) &&
                                 (states_i >= a.n_states_samples_remove) && sim_ok;
-            const double* nz = a.noise + (long long)sh_t * a.inner_it * N * M::n_noise;
+            // this bw step's noise rows as one buffer: the loads below take
+            // the row offset in a scalar register and the node's offset in a
+            // vector one, with no vector address arithmetic
+            const int row_bytes = N * M::n_noise * (int)sizeof(double);
+            const __amdgpu_buffer_rsrc_t nbuf = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<double*>(a.noise) + (long long)sh_t * a.inner_it * N * M::n_noise, 0,
+                a.inner_it * row_bytes, kBufferConfig);
             for (int inner_i = 0; inner_i < a.inner_it; inner_i++) {
                 // this step's noise: issued before the coupling product so the
                 // loads complete under the MFMAs
-                // (a wave-uniform row address plus a 32-bit lane offset: the
-                // loads take the row in scalar registers, no vector address math)
                 double n0[NS], n1[NS];
-                const char* nrow = reinterpret_cast<const char*>(nz + (long long)inner_i * N * M::n_noise);
 #pragma unroll
                 for (int sl = 0; sl < NS; sl++) {
-                    const double* p = reinterpret_cast<const double*>(nrow + shb[sl]);
-                    n0[sl] = p[0];
-                    n1[sl] = (M::n_noise > 1) ? p[1] : 0.0;
+                    if constexpr (M::n_noise > 1) {
+                        const v4i v = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)shb[sl], inner_i * row_bytes, 0);
+                        n0[sl] = __builtin_bit_cast(double, (v2i){v[0], v[1]});
+                        n1[sl] = __builtin_bit_cast(double, (v2i){v[2], v[3]});
+                    } else {
+                        const v2i v = __builtin_amdgcn_raw_buffer_load_b64(nbuf, (int)shb[sl], inner_i * row_bytes, 0);
+                        n0[sl] = __builtin_bit_cast(double, v);
+                        n1[sl] = 0.0;
+                    }
                 }
                 // the part of the model step that does not read this step's
                 // coupling input, placed before the product so that the
