@@ -172,6 +172,21 @@ def test_batched_node_counts(core, nodes):
     _compare(out, ref, "rWW")
 
 
+@pytest.mark.parametrize("nodes", [33, 100])
+@pytest.mark.parametrize("dt", ["0.2", "1.0"])
+def test_odd_inner_step_count(core, nodes, dt):
+    """An odd number of integration steps per Balloon-Windkessel step (5 at
+    dt 0.2 ms, 1 at dt 1 ms): the batched integrator's two X buffers then
+    alternate across bw steps as well as inside them."""
+    S = 16
+    sc = cases.synthetic_sc(nodes, seed=nodes + 3)
+    gp, rp = cases.params("rWW", S, nodes, G=np.linspace(0.2, 2.0, S), do_fic=False, wIE=1.0)
+    cfg = cases.model_config("rWW", do_fic=False, TR=1.0, window_size=4, window_step=2, bold_remove_s=2)
+    args = cases.sim_args("rWW", sc, gp, rp, cfg, S, duration_s=10, dt=dt)
+    out, ref, _, _ = _run_both(core, args)
+    _compare(out, ref, "rWW")
+
+
 def test_rwwex_c1_shape(core):
     """rWWEx on the C1 shape (N = 100, 60 s, 8 simulations over G)."""
     N = 100
