@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
         for (int i = 0; i < M::n_global; i++) gp[i] = a.gp[(long long)i * a.N_SIMS + simr];
 
         double s[NS][M::n_state], rp[NS][M::n_regional];
-        double bx[NS], bf[NS], bn[NS], bq[NS], sn[NS], cs[NS];
+        double bx[NS], bf[NS], bn[NS], bq[NS], bnp[NS], sn[NS], cs[NS];
         unsigned shb[NS];  // byte offset of the node's noise in a step's row
         // zero both X buffers (padding rows stay finite: A is zero there)
         for (int i = threadIdx.x; i < 2 * buf_stride; i += blockDim.x) xs[i] = 0.0;
@@ -125,6 +125,7 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
             M::init(mc, s[sl], rp[sl]);
             M::scale_rp(mc, rp[sl]);  // the split steps' form of the parameters
             bx[sl] = 0.0; bf[sl] = 1.0; bn[sl] = 1.0; bq[sl] = 1.0;
+            bnp[sl] = bw_nu_pow(a.bw, 1.0);  // nu^(1/alpha) of the initial nu
             sn[sl] = 0.0; cs[sl] = 1.0;
             shb[sl] = (unsigned)a.sh_nodes[jr] * (unsigned)(sizeof(double) * M::n_noise);
             if (j < N) publish<M>(xs, KNP, j, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
@@ -286,7 +287,15 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
                 buf = 1;
             }
 #pragma unroll
-            for (int sl = 0; sl < NS; sl++) bw_step_fast(a.bw, s[sl][M::bold_idx], bx[sl], bf[sl], bn[sl], bq[sl], ex);
+            for (int sl = 0; sl < NS; sl++) {
+                // carrying nu^(1/alpha) costs a register pair per slot: only
+                // where the slots leave room (one tile per wave, and not the
+                // JR and WC models with their 6 and 8 regional parameters)
+                if constexpr (TPW == 1 && M::n_state + 2 * M::n_regional <= 14)
+                    bw_step_carry(a.bw, s[sl][M::bold_idx], bx[sl], bf[sl], bn[sl], bq[sl], bnp[sl], ex);
+                else
+                    bw_step_fast(a.bw, s[sl][M::bold_idx], bx[sl], bf[sl], bn[sl], bq[sl], ex);
+            }
             if ((bw_i + 1) % a.BOLD_TR_iters == 0) {
                 if (sim_ok && bold_i < a.bold_len)
 #pragma unroll

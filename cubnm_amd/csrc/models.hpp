@@ -409,6 +409,22 @@ __device__ inline void bw_step_fast(const BWConsts& b, double s, double& x, doub
     f = tmp_f;
 }
 
+// bw_step_fast with nu^(1/alpha) carried between steps: the power of the old
+// nu that the nu update reads is the power of the new nu that the previous
+// step's q update computed, so one bw_nu_pow per step instead of two (the
+// same values bit for bit).  nup enters as bw_nu_pow(nu) and leaves as the
+// power of the updated nu.
+template <class E = ExpLib>
+__device__ inline void bw_step_carry(const BWConsts& b, double s, double& x, double& f, double& nu, double& q,
+                                     double& nup, const E& ex = E()) {
+    x = x + b.dt * (s - b.kappa * x - b.y * (f - 1.0));
+    const double tmp_f = f + b.dt * x;
+    nu = nu + b.dt_itau * (f - nup);
+    nup = bw_nu_pow(b, nu);
+    q = q + b.dt_itau * (f * (1.0 - ex(fast_div(b.log_oneminrho, f))) * b.irho - fast_div(nup * q, nu));
+    f = tmp_f;
+}
+
 __device__ inline void bw_step_explog(const BWConsts& b, double s, double& x, double& f, double& nu, double& q) {
     x = x + b.dt * (s - b.kappa * x - b.y * (f - 1.0));
     const double tmp_f = f + b.dt * x;
