@@ -155,123 +155,123 @@ __global__ void __launch_bounds__(64 * MAXW) integrate_batched_kernel(BatchedArg
             // constant, so the LDS addresses of both buffers are immediate offsets
             auto inner_step = [&](auto BUF, const int inner_i) __attribute__((always_inline)) {
                 constexpr int B = decltype(BUF)::value;
-                    // this step's noise: issued before the coupling product so the
-                    // loads complete under the MFMAs
-                    double n0[NS], n1[NS];
+                // this step's noise: issued before the coupling product so the
+                // loads complete under the MFMAs
+                double n0[NS], n1[NS];
 #pragma unroll
-                    for (int sl = 0; sl < NS; sl++) {
-                        if constexpr (M::n_noise > 1) {
-                            const v4i v = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)shb[sl], inner_i * row_bytes, 0);
-                            n0[sl] = __builtin_bit_cast(double, (v2i){v[0], v[1]});
-                            n1[sl] = __builtin_bit_cast(double, (v2i){v[2], v[3]});
-                        } else {
-                            const v2i v = __builtin_amdgcn_raw_buffer_load_b64(nbuf, (int)shb[sl], inner_i * row_bytes, 0);
-                            n0[sl] = __builtin_bit_cast(double, v);
-                            n1[sl] = 0.0;
-                        }
+                for (int sl = 0; sl < NS; sl++) {
+                    if constexpr (M::n_noise > 1) {
+                        const v4i v = __builtin_amdgcn_raw_buffer_load_b128(nbuf, (int)shb[sl], inner_i * row_bytes, 0);
+                        n0[sl] = __builtin_bit_cast(double, (v2i){v[0], v[1]});
+                        n1[sl] = __builtin_bit_cast(double, (v2i){v[2], v[3]});
+                    } else {
+                        const v2i v = __builtin_amdgcn_raw_buffer_load_b64(nbuf, (int)shb[sl], inner_i * row_bytes, 0);
+                        n0[sl] = __builtin_bit_cast(double, v);
+                        n1[sl] = 0.0;
                     }
-                    // the part of the model step that does not read this step's
-                    // coupling input, placed before the product so that the
-                    // scheduler can issue it between the MFMAs
-                    double tp[NS][M::n_tmp];
+                }
+                // the part of the model step that does not read this step's
+                // coupling input, placed before the product so that the
+                // scheduler can issue it between the MFMAs
+                double tp[NS][M::n_tmp];
 #pragma unroll
-                    for (int sl = 0; sl < NS; sl++) M::pre(mc, s[sl], gp, rp[sl], n0[sl], n1[sl], tp[sl], ex);
-                    const double* xb = xs + B * buf_stride;
-                    // two accumulators per product (even and odd source
-                    // fragments), added at the end: two independent MFMA chains
-                    // instead of one dependent one
-                    v4d acc[TPW][NX], acc2[TPW][NX];
+                for (int sl = 0; sl < NS; sl++) M::pre(mc, s[sl], gp, rp[sl], n0[sl], n1[sl], tp[sl], ex);
+                const double* xb = xs + B * buf_stride;
+                // two accumulators per product (even and odd source
+                // fragments), added at the end: two independent MFMA chains
+                // instead of one dependent one
+                v4d acc[TPW][NX], acc2[TPW][NX];
 #pragma unroll
-                    for (int t = 0; t < TPW; t++)
+                for (int t = 0; t < TPW; t++)
+#pragma unroll
+                    for (int x = 0; x < NX; x++) {
+                        acc[t][x] = v4d{0.0, 0.0, 0.0, 0.0};
+                        acc2[t][x] = v4d{0.0, 0.0, 0.0, 0.0};
+                    }
+                const double* ab = abase + (long long)(wave * TPW) * KT * 64 + lane;
+                const double* bb = xb + lane;
+                if constexpr (KTC > 0) {
+                    // compile-time fragment count: fully unrolled, immediate LDS offsets
+#pragma unroll
+                    for (int kt = 0; kt < KTU; kt++) {
 #pragma unroll
                         for (int x = 0; x < NX; x++) {
-                            acc[t][x] = v4d{0.0, 0.0, 0.0, 0.0};
-                            acc2[t][x] = v4d{0.0, 0.0, 0.0, 0.0};
-                        }
-                    const double* ab = abase + (long long)(wave * TPW) * KT * 64 + lane;
-                    const double* bb = xb + lane;
-                    if constexpr (KTC > 0) {
-                        // compile-time fragment count: fully unrolled, immediate LDS offsets
+                            const double b = bb[x * KNP * 16 + kt * 64];
 #pragma unroll
-                        for (int kt = 0; kt < KTU; kt++) {
-#pragma unroll
-                            for (int x = 0; x < NX; x++) {
-                                const double b = bb[x * KNP * 16 + kt * 64];
-#pragma unroll
-                                for (int t = 0; t < TPW; t++) {
-                                    if (kt & 1)
-                                        acc2[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[(t * KT + kt) * 64], b,
-                                                                                          acc2[t][x], 0, 0, 0);
-                                    else
-                                        acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[(t * KT + kt) * 64], b,
-                                                                                         acc[t][x], 0, 0, 0);
-                                }
+                            for (int t = 0; t < TPW; t++) {
+                                if (kt & 1)
+                                    acc2[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[(t * KT + kt) * 64], b,
+                                                                                      acc2[t][x], 0, 0, 0);
+                                else
+                                    acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[(t * KT + kt) * 64], b,
+                                                                                     acc[t][x], 0, 0, 0);
                             }
                         }
-                    } else {
-                        // runtime fragment count: software-pipelined two k-steps ahead
-                        double a0[TPW], a1[TPW], b0[NX], b1[NX];
-#pragma unroll
-                        for (int t = 0; t < TPW; t++) { a0[t] = ab[t * KT * 64]; a1[t] = ab[(t * KT + 1) * 64]; }
-#pragma unroll
-                        for (int x = 0; x < NX; x++) { b0[x] = bb[x * KNP * 16]; b1[x] = bb[x * KNP * 16 + 64]; }
-                        for (int kt = 0; kt < KT; kt += 2) {
-                            const int k2 = min(kt + 2, KT - 2), k3 = k2 + 1;
-                            double a2[TPW], a3[TPW], b2[NX], b3[NX];
-#pragma unroll
-                            for (int t = 0; t < TPW; t++) { a2[t] = ab[(t * KT + k2) * 64]; a3[t] = ab[(t * KT + k3) * 64]; }
-#pragma unroll
-                            for (int x = 0; x < NX; x++) { b2[x] = bb[x * KNP * 16 + k2 * 64]; b3[x] = bb[x * KNP * 16 + k3 * 64]; }
-                            // keep the prefetch ahead of the MFMAs consuming the previous pair
-                            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                            for (int x = 0; x < NX; x++)
-#pragma unroll
-                                for (int t = 0; t < TPW; t++) {
-                                    acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0[x], acc[t][x], 0, 0, 0);
-                                    acc2[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1[x], acc2[t][x], 0, 0, 0);
-                                }
-                            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                            for (int t = 0; t < TPW; t++) { a0[t] = a2[t]; a1[t] = a3[t]; }
-#pragma unroll
-                            for (int x = 0; x < NX; x++) { b0[x] = b2[x]; b1[x] = b3[x]; }
-                        }
                     }
+                } else {
+                    // runtime fragment count: software-pipelined two k-steps ahead
+                    double a0[TPW], a1[TPW], b0[NX], b1[NX];
 #pragma unroll
-                    for (int t = 0; t < TPW; t++)
+                    for (int t = 0; t < TPW; t++) { a0[t] = ab[t * KT * 64]; a1[t] = ab[(t * KT + 1) * 64]; }
 #pragma unroll
-                        for (int x = 0; x < NX; x++) acc[t][x] += acc2[t][x];
-                    double* xn = xs + (B ^ 1) * buf_stride;
+                    for (int x = 0; x < NX; x++) { b0[x] = bb[x * KNP * 16]; b1[x] = bb[x * KNP * 16 + 64]; }
+                    for (int kt = 0; kt < KT; kt += 2) {
+                        const int k2 = min(kt + 2, KT - 2), k3 = k2 + 1;
+                        double a2[TPW], a3[TPW], b2[NX], b3[NX];
 #pragma unroll
-                    for (int t = 0; t < TPW; t++)
+                        for (int t = 0; t < TPW; t++) { a2[t] = ab[(t * KT + k2) * 64]; a3[t] = ab[(t * KT + k3) * 64]; }
 #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const int sl = 4 * t + r;
-                            double gin;
-                            if constexpr (M::is_osc) gin = cs[sl] * acc[t][0][r] - sn[sl] * acc[t][1][r];
-                            else gin = acc[t][0][r];
-                            M::post(mc, s[sl], gp, rp[sl], gin, n0[sl], n1[sl], tp[sl], ex);
-                        }
-                    // every slot writes its coupling state, padding rows into the
-                    // spare row: no branch between the slots' steps, so they form
-                    // one block whose dependent chains the scheduler interleaves
+                        for (int x = 0; x < NX; x++) { b2[x] = bb[x * KNP * 16 + k2 * 64]; b3[x] = bb[x * KNP * 16 + k3 * 64]; }
+                        // keep the prefetch ahead of the MFMAs consuming the previous pair
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int x = 0; x < NX; x++)
+#pragma unroll
+                            for (int t = 0; t < TPW; t++) {
+                                acc[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0[x], acc[t][x], 0, 0, 0);
+                                acc2[t][x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1[x], acc2[t][x], 0, 0, 0);
+                            }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int t = 0; t < TPW; t++) { a0[t] = a2[t]; a1[t] = a3[t]; }
+#pragma unroll
+                        for (int x = 0; x < NX; x++) { b0[x] = b2[x]; b1[x] = b3[x]; }
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < TPW; t++)
+#pragma unroll
+                    for (int x = 0; x < NX; x++) acc[t][x] += acc2[t][x];
+                double* xn = xs + (B ^ 1) * buf_stride;
+#pragma unroll
+                for (int t = 0; t < TPW; t++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int sl = 4 * t + r;
+                        double gin;
+                        if constexpr (M::is_osc) gin = cs[sl] * acc[t][0][r] - sn[sl] * acc[t][1][r];
+                        else gin = acc[t][0][r];
+                        M::post(mc, s[sl], gp, rp[sl], gin, n0[sl], n1[sl], tp[sl], ex);
+                    }
+                // every slot writes its coupling state, padding rows into the
+                // spare row: no branch between the slots' steps, so they form
+                // one block whose dependent chains the scheduler interleaves
+#pragma unroll
+                for (int sl = 0; sl < NS; sl++) {
+                    const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
+                    publish<M>(xn, KNP, j < N ? j : KN, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
+                }
+                if (sample && inner_i == a.inner_it - 1) {
 #pragma unroll
                     for (int sl = 0; sl < NS; sl++) {
                         const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
-                        publish<M>(xn, KNP, j < N ? j : KN, c, s[sl][M::conn_idx], &sn[sl], &cs[sl]);
-                    }
-                    if (sample && inner_i == a.inner_it - 1) {
+                        if (j < N)
 #pragma unroll
-                        for (int sl = 0; sl < NS; sl++) {
-                            const int j = 16 * (wave * TPW + sl / 4) + 4 * (sl & 3) + q;
-                            if (j < N)
-#pragma unroll
-                                for (int i = 0; i < M::n_state; i++)
-                                    a.states[(long long)i * a.N_SIMS * N + (long long)sim * N + j] += s[sl][i];
-                        }
+                            for (int i = 0; i < M::n_state; i++)
+                                a.states[(long long)i * a.N_SIMS * N + (long long)sim * N + j] += s[sl][i];
                     }
-                    __syncthreads();
+                }
+                __syncthreads();
             };
             using B0 = std::integral_constant<int, 0>;
             using B1 = std::integral_constant<int, 1>;
